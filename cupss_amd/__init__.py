@@ -1,0 +1,231 @@
+"""cupss_amd -- Python binding of libcupss's C-ABI (include/cupss_hip.h).
+
+The library is the MI355X HIP implementation of cuPSS's per-timestep
+pseudospectral update; this module only loads it with ctypes and mirrors the
+reference's system-building calls (evolver::createField / createTerm /
+addNoise / advanceTime, src/evolver.cpp) for tests, benchmarks and scripts.
+There is no CPU fallback: if libcupss.so is missing or no GPU is present,
+the calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcupss.so")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+
+class CupssError(RuntimeError):
+    pass
+
+
+class Pres(ctypes.Structure):
+    """cupss_pres: same layout as the reference `pres` (inc/cupss/defines.h:31-39)."""
+
+    _fields_ = [("preFactor", ctypes.c_float), ("q2n", ctypes.c_int), ("iqx", ctypes.c_int),
+                ("iqy", ctypes.c_int), ("iqz", ctypes.c_int), ("invq", ctypes.c_int)]
+
+
+def pres(pre=1.0, q2n=0, iqx=0, iqy=0, iqz=0, invq=0):
+    return Pres(float(pre), int(q2n), int(iqx), int(iqy), int(iqz), int(invq))
+
+
+_lib = None
+
+_VP = ctypes.c_void_p
+_I = ctypes.c_int
+_F = ctypes.c_float
+_FP = ctypes.POINTER(ctypes.c_float)
+_SIGS = {
+    "cupss_last_error": (ctypes.c_char_p, []),
+    "cupss_abi_version": (_I, []),
+    "cupss_system_create": (_I, [_I, _I, _I, _F, _F, _F, _F, ctypes.POINTER(_VP)]),
+    "cupss_system_destroy": (_I, [_VP]),
+    "cupss_field_create": (_I, [_VP, _I, ctypes.POINTER(_I)]),
+    "cupss_field_add_implicit": (_I, [_VP, _I, Pres]),
+    "cupss_field_set_implicits": (_I, [_VP, _I, ctypes.POINTER(Pres), _I]),
+    "cupss_term_create": (_I, [_VP, _I, ctypes.POINTER(_I), _I, ctypes.POINTER(Pres), _I, ctypes.POINTER(_I)]),
+    "cupss_term_set_prefactors": (_I, [_VP, _I, ctypes.POINTER(Pres), _I]),
+    "cupss_field_set_noise": (_I, [_VP, _I, Pres, ctypes.c_uint64]),
+    "cupss_field_set_hooks": (_I, [_VP, _I, _VP, _VP, _VP]),
+    "cupss_field_set_real": (_I, [_VP, _I, _FP]),
+    "cupss_field_get_real": (_I, [_VP, _I, _FP]),
+    "cupss_field_get_dealiased": (_I, [_VP, _I, _FP]),
+    "cupss_field_get_spectrum": (_I, [_VP, _I, _FP]),
+    "cupss_field_set_spectrum": (_I, [_VP, _I, _FP]),
+    "cupss_system_half_bins": (ctypes.c_long, [_VP]),
+    "cupss_system_prepare": (_I, [_VP]),
+    "cupss_field_aliasing_order": (_I, [_VP, _I]),
+    "cupss_system_advance": (_I, [_VP, _I]),
+    "cupss_system_time_advance": (_I, [_VP, _I, _FP]),
+    "cupss_system_synchronize": (_I, [_VP]),
+    "cupss_system_get_step": (_I, [_VP, ctypes.POINTER(ctypes.c_uint64)]),
+    "cupss_system_set_step": (_I, [_VP, ctypes.c_uint64]),
+    "cupss_system_set_stream": (_I, [_VP, _VP]),
+    "cupss_system_set_dt": (_I, [_VP, _F]),
+    "cupss_field_spectrum_device": (_I, [_VP, _I, ctypes.POINTER(_VP)]),
+    "cupss_r2c": (_I, [_I, _I, _I, _VP, _VP, _VP]),
+    "cupss_c2r": (_I, [_I, _I, _I, _VP, _I, _F, _VP, _VP, _VP]),
+    "cupss_normals": (_I, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, _VP, ctypes.c_long, _VP]),
+    "cupss_device_alloc": (_I, [ctypes.c_size_t, ctypes.POINTER(_VP)]),
+    "cupss_device_free": (_I, [_VP]),
+    "cupss_memcpy_h2d": (_I, [_VP, _VP, ctypes.c_size_t]),
+    "cupss_memcpy_d2h": (_I, [_VP, _VP, ctypes.c_size_t]),
+    "cupss_device_synchronize": (_I, []),
+    "cupss_device_count": (_I, [ctypes.POINTER(_I)]),
+}
+
+
+def lib():
+    """Load libcupss.so (built by cupss_amd/build.py).  Raises if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CupssError("libcupss.so is not built: run `python -m cupss_amd.build` (no CPU fallback exists)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise CupssError("libcupss error %d: %s" % (rc, lib().cupss_last_error().decode()))
+    return rc
+
+
+def _fptr(a):
+    return a.ctypes.data_as(_FP)
+
+
+def gpu_count():
+    n = ctypes.c_int(0)
+    rc = lib().cupss_device_count(ctypes.byref(n))
+    return n.value if rc == 0 else 0
+
+
+class DeviceBuffer:
+    """A raw device allocation made by the library (hipMalloc)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = ctypes.c_void_p()
+        check(lib().cupss_device_alloc(self.nbytes, ctypes.byref(self.ptr)))
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        check(lib().cupss_memcpy_h2d(self.ptr, arr.ctypes.data_as(_VP), arr.nbytes))
+
+    def download(self, dtype, count):
+        out = np.empty(count, dtype)
+        check(lib().cupss_memcpy_d2h(out.ctypes.data_as(_VP), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().cupss_device_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class System:
+    """One system on the GPU: the device side of one reference evolver."""
+
+    def __init__(self, sx, sy=1, sz=1, dx=1.0, dy=1.0, dz=1.0, dt=0.01):
+        self.L = lib()
+        self.h = ctypes.c_void_p()
+        check(self.L.cupss_system_create(sx, sy, sz, dx, dy, dz, dt, ctypes.byref(self.h)))
+        self.dims = (sx, sy, sz)
+        self.n = sx * sy * sz
+        self.nh = (sx // 2 + 1) * sy * sz
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.L.cupss_system_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_field(self, dynamic):
+        fid = ctypes.c_int()
+        check(self.L.cupss_field_create(self.h, int(bool(dynamic)), ctypes.byref(fid)))
+        return fid.value
+
+    def add_implicit(self, f, p):
+        check(self.L.cupss_field_add_implicit(self.h, f, p))
+
+    def set_implicits(self, f, prefs):
+        arr = (Pres * max(1, len(prefs)))(*prefs)
+        check(self.L.cupss_field_set_implicits(self.h, f, arr, len(prefs)))
+
+    def add_term(self, f, factors, prefs):
+        fac = (ctypes.c_int * max(1, len(factors)))(*factors)
+        arr = (Pres * len(prefs))(*prefs)
+        tid = ctypes.c_int()
+        check(self.L.cupss_term_create(self.h, f, fac, len(factors), arr, len(prefs), ctypes.byref(tid)))
+        return tid.value
+
+    def set_term_prefactors(self, t, prefs):
+        arr = (Pres * len(prefs))(*prefs)
+        check(self.L.cupss_term_set_prefactors(self.h, t, arr, len(prefs)))
+
+    def set_noise(self, f, amp, seed):
+        check(self.L.cupss_field_set_noise(self.h, f, amp, seed))
+
+    def set_real(self, f, x):
+        x = np.ascontiguousarray(np.asarray(x, np.float32).ravel())
+        assert x.size == self.n
+        check(self.L.cupss_field_set_real(self.h, f, _fptr(x)))
+
+    def prepare(self):
+        check(self.L.cupss_system_prepare(self.h))
+
+    def step(self, n=1):
+        check(self.L.cupss_system_advance(self.h, n))
+
+    def time_steps(self, n):
+        ms = ctypes.c_float()
+        check(self.L.cupss_system_time_advance(self.h, n, ctypes.byref(ms)))
+        return ms.value
+
+    def synchronize(self):
+        check(self.L.cupss_system_synchronize(self.h))
+
+    def get_real(self, f):
+        out = np.empty(self.n, np.float32)
+        check(self.L.cupss_field_get_real(self.h, f, _fptr(out)))
+        return out
+
+    def get_dealiased(self, f):
+        out = np.empty(self.n, np.float32)
+        check(self.L.cupss_field_get_dealiased(self.h, f, _fptr(out)))
+        return out
+
+    def get_spectrum(self, f):
+        out = np.empty(2 * self.nh, np.float32)
+        check(self.L.cupss_field_get_spectrum(self.h, f, _fptr(out)))
+        return out[0::2] + 1j * out[1::2]
+
+    def aliasing_order(self, f):
+        return self.L.cupss_field_aliasing_order(self.h, f)
+
+    @property
+    def step_count(self):
+        v = ctypes.c_uint64()
+        check(self.L.cupss_system_get_step(self.h, ctypes.byref(v)))
+        return v.value

@@ -1,0 +1,192 @@
+// fft_core.hpp -- device building blocks for the LDS-resident Stockham FFTs.
+//
+// A line of N complex values is held by T = N/E threads of one workgroup,
+// E values per thread in the "strided" distribution: thread t owns
+// x[t + e*T], e = 0..E-1.  Each Stockham stage of radix R (R <= E, R | E)
+// does E/R radix-R butterflies per thread in registers, writes the outputs
+// to the line's LDS region at their Stockham positions and reads back the
+// strided distribution for the next stage.  After the last stage the
+// transform is in natural order both in LDS and in the registers
+// (v[e] = X[t + e*T]).  Radix schedule: stages of radix E while the
+// remaining length allows it, then one stage with the remainder.
+//
+// Twiddles come from a table of exp(-2*pi*i*j/TW_N), j < TW_N, computed in
+// double precision on the host and rounded to float (twiddle_table()).
+//
+// Sign convention (the reference's cuFFT/FFTW C2C, unnormalised):
+// DIR = -1 forward, exp(-2*pi*i*k*n/N); DIR = +1 inverse.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace cupss {
+
+constexpr int TW_LOG = 13;
+constexpr int TW_N = 1 << TW_LOG;  // largest supported line length (complex points)
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+// multiply by i^DIR: DIR = +1 -> i, DIR = -1 -> -i
+template <int DIR>
+__device__ __forceinline__ float2 mul_i(float2 a)
+{
+    return DIR > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// cos(2*pi*j/32), j = 0..31, and the matching sines, for the in-register
+// DFT twiddles (R <= 32).
+__device__ constexpr float kC32[32] = {
+    1.0f, 0.98078528040323044f, 0.92387953251128674f, 0.83146961230254524f, 0.70710678118654752f,
+    0.55557023301960218f, 0.38268343236508977f, 0.19509032201612826f, 0.0f, -0.19509032201612826f,
+    -0.38268343236508977f, -0.55557023301960218f, -0.70710678118654752f, -0.83146961230254524f,
+    -0.92387953251128674f, -0.98078528040323044f, -1.0f, -0.98078528040323044f, -0.92387953251128674f,
+    -0.83146961230254524f, -0.70710678118654752f, -0.55557023301960218f, -0.38268343236508977f,
+    -0.19509032201612826f, 0.0f, 0.19509032201612826f, 0.38268343236508977f, 0.55557023301960218f,
+    0.70710678118654752f, 0.83146961230254524f, 0.92387953251128674f, 0.98078528040323044f};
+
+// a * exp(DIR*2*pi*i*k/R) with compile-time k, R
+template <int R, int K, int DIR>
+__device__ __forceinline__ float2 twc(float2 a)
+{
+    constexpr int j = (K * (32 / R)) & 31;
+    if constexpr (j == 0) {
+        return a;
+    } else if constexpr (j == 8) {
+        return mul_i<DIR>(a);
+    } else if constexpr (j == 16) {
+        return make_float2(-a.x, -a.y);
+    } else if constexpr (j == 24) {
+        return mul_i<-DIR>(a);
+    } else {
+        constexpr float c = kC32[j];
+        constexpr float s = kC32[(j + 24) & 31] * (float)DIR;  // sin(2 pi j/32) * DIR
+        return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+    }
+}
+
+// In-register radix-R DFT, natural order in and out:
+//   a[k] <- sum_n a[n] exp(DIR*2*pi*i*k*n/R).
+template <int R, int DIR>
+struct Dft {
+    template <int K>
+    __device__ __forceinline__ static void combine(float2 *a, const float2 *e, const float2 *o)
+    {
+        if constexpr (K < R / 2) {
+            float2 t = twc<R, K, DIR>(o[K]);
+            a[K] = cadd(e[K], t);
+            a[K + R / 2] = csub(e[K], t);
+            combine<K + 1>(a, e, o);
+        }
+    }
+    __device__ __forceinline__ static void run(float2 *a)
+    {
+        float2 e[R / 2], o[R / 2];
+#pragma unroll
+        for (int n = 0; n < R / 2; n++) {
+            e[n] = a[2 * n];
+            o[n] = a[2 * n + 1];
+        }
+        Dft<R / 2, DIR>::run(e);
+        Dft<R / 2, DIR>::run(o);
+        combine<0>(a, e, o);
+    }
+};
+template <int DIR>
+struct Dft<1, DIR> {
+    __device__ __forceinline__ static void run(float2 *) {}
+};
+template <int DIR>
+struct Dft<2, DIR> {
+    __device__ __forceinline__ static void run(float2 *a)
+    {
+        float2 t = a[1];
+        a[1] = csub(a[0], t);
+        a[0] = cadd(a[0], t);
+    }
+};
+template <int DIR>
+struct Dft<4, DIR> {
+    __device__ __forceinline__ static void run(float2 *a)
+    {
+        float2 t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
+        float2 t2 = cadd(a[1], a[3]), t3 = mul_i<DIR>(csub(a[1], a[3]));
+        a[0] = cadd(t0, t2);
+        a[2] = csub(t0, t2);
+        a[1] = cadd(t1, t3);
+        a[3] = csub(t1, t3);
+    }
+};
+
+// LDS index with padding: one extra slot every 32 entries, which breaks the
+// power-of-two strides of the Stockham writes across banks.
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 5); }
+__host__ __device__ constexpr int lds_len(int n) { return n + (n >> 5) + 1; }
+
+constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
+
+// elements per thread for a line of n points
+__host__ __device__ constexpr int elems_for(int n) { return n >= 256 ? 16 : (n >= 16 ? 8 : n); }
+
+// exp(DIR*2*pi*i*m/L) from the table (L | TW_N, 0 <= m)
+template <int DIR>
+__device__ __forceinline__ float2 tw_lookup(const float2 *__restrict__ tw, int m, int L)
+{
+    int j = (m * (TW_N / L)) & (TW_N - 1);
+    float2 w = tw[j];
+    return DIR > 0 ? make_float2(w.x, -w.y) : w;
+}
+
+// One Stockham stage of radix R with span NS, then recurse.
+template <int N, int E, int DIR, int NS>
+struct Stockham {
+    static constexpr int T = N / E;
+    static constexpr int REM = N / NS;
+    static constexpr int R = REM >= E ? E : REM;
+    __device__ __forceinline__ static void run(float2 (&v)[E], float2 *lds, int t, const float2 *__restrict__ tw)
+    {
+        if constexpr (NS < N) {
+#pragma unroll
+            for (int b = 0; b < E / R; b++) {
+                float2 a[R];
+#pragma unroll
+                for (int r = 0; r < R; r++) a[r] = v[b + r * (E / R)];
+                const int j = t + b * T;
+                const int k = j & (NS - 1);
+                if constexpr (NS > 1) {
+#pragma unroll
+                    for (int r = 1; r < R; r++) a[r] = cmul(a[r], tw_lookup<DIR>(tw, r * k, NS * R));
+                }
+                Dft<R, DIR>::run(a);
+                const int base = (j - k) * R + k;
+#pragma unroll
+                for (int r = 0; r < R; r++) lds[lpad(base + r * NS)] = a[r];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < E; e++) v[e] = lds[lpad(t + e * T)];
+            if constexpr (NS * R < N) __syncthreads();
+            Stockham<N, E, DIR, NS * R>::run(v, lds, t, tw);
+        }
+    }
+};
+
+// Full transform of one line.  On entry v[e] = x[t + e*T]; on exit
+// v[e] = X[t + e*T] and lds[lpad(k)] = X[k] for all k (after a barrier).
+// All threads of the workgroup must call it (it contains barriers).
+template <int N, int E, int DIR>
+__device__ __forceinline__ void fft_line(float2 (&v)[E], float2 *lds, int t, const float2 *__restrict__ tw)
+{
+    if constexpr (N == 1) {
+        lds[0] = v[0];
+        __syncthreads();
+    } else {
+        Stockham<N, E, DIR, 1>::run(v, lds, t, tw);
+    }
+}
+
+}  // namespace cupss

@@ -1,0 +1,502 @@
+// kernels.hip -- the per-pass FFT kernels of the plain (per-solve) path and
+// the fused semi-implicit update.
+//
+// Layout: real fields are float[sz][sy][sx] (x fastest, the reference's
+// idx = k*sx*sy + j*sx + i); spectra are the Hermitian half
+// float2[sz][sy][sx/2+1].  The C2C passes along y and z work in place on
+// the half spectrum.
+//
+//   forward (R2C):  rows_r2c (x, with a fused source: a real field, the
+//                   product of several real fields, or Philox noise)
+//                   -> cols y -> cols z
+//   inverse (C2R):  cols z (first pass carries the dealias mask) -> cols y
+//                   -> rows_c2r (x, 1/N scaling; mask here in 1D)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+#include <cmath>
+
+#include "fft_core.hpp"
+#include "internal.hpp"
+#include "philox.hpp"
+
+namespace cupss {
+
+// ------------------------------------------------------------ twiddles
+
+static std::mutex g_tw_mutex;
+static std::vector<float2 *> g_tw;  // per device
+
+const float2 *twiddles(hipStream_t s)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lock(g_tw_mutex);
+    if ((int)g_tw.size() <= dev) g_tw.resize(dev + 1, nullptr);
+    if (!g_tw[dev]) {
+        std::vector<float2> h(TW_N);
+        for (int j = 0; j < TW_N; j++) {
+            double a = -2.0 * M_PI * (double)j / (double)TW_N;
+            h[j] = make_float2((float)std::cos(a), (float)std::sin(a));
+        }
+        float2 *d = nullptr;
+        if (hipMalloc(&d, sizeof(float2) * TW_N) != hipSuccess) return nullptr;
+        if (hipMemcpy(d, h.data(), sizeof(float2) * TW_N, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+        g_tw[dev] = d;
+    }
+    (void)s;
+    return g_tw[dev];
+}
+
+// ------------------------------------------------------------ sources
+
+struct SrcArgs {
+    const float *f[MAX_FACTORS];
+    int nf;
+    uint32_t key0, key1;
+    uint32_t step_lo, step_hi, stream;
+};
+
+// z[m] = x[2m] + i x[2m+1] of row `row` (real row length 2*nc)
+template <int KIND>
+__device__ __forceinline__ float2 load_src(const SrcArgs &a, long row, int m, int nc)
+{
+    const long idx = row * (2L * nc) + 2L * m;
+    if constexpr (KIND == LOAD_REAL) {
+        return *reinterpret_cast<const float2 *>(a.f[0] + idx);
+    } else if constexpr (KIND == LOAD_PRODUCT) {
+        float2 r = make_float2(1.0f, 1.0f);
+        for (int c = 0; c < a.nf; c++) {
+            float2 v = *reinterpret_cast<const float2 *>(a.f[c] + idx);
+            r.x *= v.x;
+            r.y *= v.y;
+        }
+        return r;
+    } else {
+        // two consecutive cells share one Philox block (idx even)
+        float2 z = philox_normal_pair((uint32_t)idx, a.step_lo, a.step_hi, a.stream, a.key0, a.key1);
+        return z;
+    }
+}
+
+// ------------------------------------------------------------ rows R2C
+
+template <int NC>
+struct RowCfg {
+    static constexpr int E = elems_for(NC);
+    static constexpr int T = NC / E;
+    static constexpr int LPB = T >= 256 ? 1 : 256 / T;
+    static constexpr int THREADS = LPB * T;
+    static constexpr int LL = lds_len(NC);
+};
+
+template <int NC, int KIND>
+__global__ __launch_bounds__(256) void k_rows_r2c(SrcArgs src, float2 *__restrict__ out, long nrows,
+                                                  const float2 *__restrict__ tw)
+{
+    using C = RowCfg<NC>;
+    constexpr int E = C::E, T = C::T;
+    __shared__ float2 lds[C::LPB * C::LL];
+    const int l = threadIdx.x / T, t = threadIdx.x % T;
+    const long row = (long)blockIdx.x * C::LPB + l;
+    const bool active = row < nrows;
+    float2 *L = lds + l * C::LL;
+    float2 v[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) v[e] = active ? load_src<KIND>(src, row, t + e * T, NC) : make_float2(0.f, 0.f);
+    fft_line<NC, E, -1>(v, L, t, tw);
+    if (!active) return;
+    float2 *o = out + row * (NC + 1);
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int k = t + e * T;
+        const float2 A = v[e];
+        const float2 B = cconj(L[lpad((NC - k) & (NC - 1))]);
+        const float2 xe = cscale(cadd(A, B), 0.5f);
+        const float2 xo = cscale(mul_i<-1>(csub(A, B)), 0.5f);  // (A-B)/(2i)
+        float2 X;
+        if (k == 0) {
+            X = make_float2(xe.x + xo.x, 0.0f);
+        } else {
+            X = cadd(xe, cmul(tw_lookup<-1>(tw, k, 2 * NC), xo));
+        }
+        o[k] = X;
+    }
+    if (t == 0) {
+        const float2 z0 = L[lpad(0)];
+        o[NC] = make_float2(z0.x - z0.y, 0.0f);
+    }
+}
+
+// ------------------------------------------------------------ rows C2R
+
+template <int NC, bool MASK>
+__global__ __launch_bounds__(256) void k_rows_c2r(const float2 *__restrict__ in, float *__restrict__ out,
+                                                  long nrows, float scale, int cut,
+                                                  const float2 *__restrict__ tw)
+{
+    using C = RowCfg<NC>;
+    constexpr int E = C::E, T = C::T;
+    __shared__ float2 lds[C::LPB * C::LL];
+    const int l = threadIdx.x / T, t = threadIdx.x % T;
+    const long row = (long)blockIdx.x * C::LPB + l;
+    const bool active = row < nrows;
+    const float2 *x = in + row * (NC + 1);
+    float2 v[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int k = t + e * T;
+        float2 X = make_float2(0.f, 0.f), Y = make_float2(0.f, 0.f);
+        if (active) {
+            X = x[k];
+            Y = x[NC - k];
+            if (k == 0) {  // self-conjugate bins: keep the real part (projection)
+                X.y = 0.f;
+                Y.y = 0.f;
+            }
+            if (MASK) {
+                if (k > cut) X = make_float2(0.f, 0.f);
+                if (NC - k > cut) Y = make_float2(0.f, 0.f);
+            }
+        }
+        const float2 Yc = cconj(Y);
+        const float2 xe = cadd(X, Yc);
+        const float2 xo = cmul(csub(X, Yc), tw_lookup<+1>(tw, k, 2 * NC));
+        v[e] = cadd(xe, mul_i<+1>(xo));
+    }
+    fft_line<NC, E, +1>(v, lds + l * C::LL, t, tw);
+    if (!active) return;
+    float2 *o = reinterpret_cast<float2 *>(out + row * (2L * NC));
+#pragma unroll
+    for (int e = 0; e < E; e++) o[t + e * T] = cscale(v[e], scale);
+}
+
+// ------------------------------------------------------------ columns
+
+template <int N>
+struct ColCfg {
+    static constexpr int E = elems_for(N);
+    static constexpr int T = N / E;
+    static constexpr int TWD = T >= 64 ? std::max(1, 512 / T) : std::min(64, std::max(1, 256 / T));
+    static constexpr int THREADS = TWD * T;
+    static constexpr int LL = lds_len(N);
+};
+
+// C2C along a strided axis.  Line `col` of batch `b` is
+//   p[b*bstride + col + y*stride], y = 0..N-1, col = 0..ncols-1.
+// MASK: dealias mask of order p on the half spectrum (first inverse pass);
+// `axis` tells which axis the lines run along (1 = y in 2D, 2 = z in 3D).
+template <int N, int DIR, bool MASK>
+__global__ __launch_bounds__(512) void k_cols(const float2 *__restrict__ in, float2 *__restrict__ out, long ncols,
+                                              long stride, long bstride, Geom g, int p,
+                                              const float2 *__restrict__ tw)
+{
+    using C = ColCfg<N>;
+    constexpr int E = C::E, T = C::T;
+    __shared__ float2 lds[C::TWD * C::LL];
+    const int c = threadIdx.x % C::TWD, t = threadIdx.x / C::TWD;
+    const long col = (long)blockIdx.x * C::TWD + c;
+    const long b = blockIdx.y;
+    const bool active = col < ncols;
+    const float2 *src = in + b * bstride + col;
+    float2 v[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) v[e] = active ? src[(long)(t + e * T) * stride] : make_float2(0.f, 0.f);
+    if constexpr (MASK) {
+        // bin indices of this line: 2D lines run along y (col = kx);
+        // 3D first pass lines run along z (col = ky*nc1 + kx)
+        const int kx = (int)(col % g.nc1);
+        const int ky0 = (int)(col / g.nc1);
+        const int p1 = p + 1;
+        const bool okx = kx <= g.sx / p1;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int y = t + e * T;
+            int nyv, nzv;
+            if (g.sz > 1) {
+                nyv = ky0;
+                nzv = y;
+            } else {
+                nyv = y;
+                nzv = 0;
+            }
+            int ny = nyv > g.sy / 2 ? nyv - g.sy : nyv;
+            int nz = nzv > g.sz / 2 ? nzv - g.sz : nzv;
+            const bool keep = okx && abs(ny) <= g.sy / p1 && abs(nz) <= g.sz / p1;
+            if (!keep) v[e] = make_float2(0.f, 0.f);
+        }
+    }
+    fft_line<N, E, DIR>(v, lds + c * C::LL, t, tw);
+    if (!active) return;
+    float2 *dst = out + b * bstride + col;
+#pragma unroll
+    for (int e = 0; e < E; e++) dst[(long)(t + e * T) * stride] = v[e];
+}
+
+// ------------------------------------------------------------ dispatch
+
+#define CUPSS_POW2_CASES(X) \
+    X(1) X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048)
+
+static hipError_t rows_r2c(const Geom &g, const SrcArgs &a, int kind, float2 *out, hipStream_t s,
+                           const float2 *tw)
+{
+    const long nrows = (long)g.sy * g.sz;
+    switch (g.nc) {
+#define X(NC)                                                                                        \
+    case NC: {                                                                                       \
+        dim3 grid((unsigned)((nrows + RowCfg<NC>::LPB - 1) / RowCfg<NC>::LPB));                      \
+        dim3 block(RowCfg<NC>::THREADS);                                                             \
+        if (kind == LOAD_REAL)                                                                       \
+            hipLaunchKernelGGL((k_rows_r2c<NC, LOAD_REAL>), grid, block, 0, s, a, out, nrows, tw);   \
+        else if (kind == LOAD_PRODUCT)                                                               \
+            hipLaunchKernelGGL((k_rows_r2c<NC, LOAD_PRODUCT>), grid, block, 0, s, a, out, nrows, tw); \
+        else                                                                                         \
+            hipLaunchKernelGGL((k_rows_r2c<NC, LOAD_NOISE>), grid, block, 0, s, a, out, nrows, tw);  \
+        return hipGetLastError();                                                                    \
+    }
+        CUPSS_POW2_CASES(X)
+#undef X
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+static hipError_t rows_c2r(const Geom &g, const float2 *in, float *out, float scale, int mask_p, hipStream_t s,
+                           const float2 *tw)
+{
+    const long nrows = (long)g.sy * g.sz;
+    const int cut = mask_p > 0 ? g.sx / (mask_p + 1) : 1 << 30;
+    switch (g.nc) {
+#define X(NC)                                                                                              \
+    case NC: {                                                                                             \
+        dim3 grid((unsigned)((nrows + RowCfg<NC>::LPB - 1) / RowCfg<NC>::LPB));                            \
+        dim3 block(RowCfg<NC>::THREADS);                                                                   \
+        if (mask_p > 0)                                                                                    \
+            hipLaunchKernelGGL((k_rows_c2r<NC, true>), grid, block, 0, s, in, out, nrows, scale, cut, tw); \
+        else                                                                                               \
+            hipLaunchKernelGGL((k_rows_c2r<NC, false>), grid, block, 0, s, in, out, nrows, scale, cut, tw); \
+        return hipGetLastError();                                                                          \
+    }
+        CUPSS_POW2_CASES(X)
+#undef X
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+#define CUPSS_COL_CASES(X) \
+    X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
+
+static hipError_t cols(const Geom &g, int n, int dir, bool mask, int p, const float2 *in, float2 *out, long ncols,
+                       long stride, long bstride, long nbatch, hipStream_t s, const float2 *tw)
+{
+    if (n == 1) {
+        if (in != out)
+            return hipMemcpyAsync(out, in, sizeof(float2) * (size_t)ncols * nbatch, hipMemcpyDeviceToDevice, s);
+        return hipSuccess;
+    }
+    switch (n) {
+#define X(N)                                                                                                   \
+    case N: {                                                                                                  \
+        dim3 grid((unsigned)((ncols + ColCfg<N>::TWD - 1) / ColCfg<N>::TWD), (unsigned)nbatch);               \
+        dim3 block(ColCfg<N>::THREADS);                                                                        \
+        if (dir < 0)                                                                                           \
+            hipLaunchKernelGGL((k_cols<N, -1, false>), grid, block, 0, s, in, out, ncols, stride, bstride, g, p, \
+                               tw);                                                                            \
+        else if (mask)                                                                                         \
+            hipLaunchKernelGGL((k_cols<N, 1, true>), grid, block, 0, s, in, out, ncols, stride, bstride, g, p,  \
+                               tw);                                                                            \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_cols<N, 1, false>), grid, block, 0, s, in, out, ncols, stride, bstride, g, p, \
+                               tw);                                                                            \
+        return hipGetLastError();                                                                              \
+    }
+        CUPSS_COL_CASES(X)
+#undef X
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+hipError_t r2c(const Geom &g, const R2CSource &src, float2 *out, hipStream_t s)
+{
+    const float2 *tw = twiddles(s);
+    if (!tw) return hipErrorOutOfMemory;
+    SrcArgs a{};
+    for (int i = 0; i < MAX_FACTORS; i++) a.f[i] = src.f[i];
+    a.nf = src.nf;
+    a.key0 = (uint32_t)src.seed;
+    a.key1 = (uint32_t)(src.seed >> 32);
+    a.step_lo = (uint32_t)src.step;
+    a.step_hi = (uint32_t)(src.step >> 32);
+    a.stream = src.stream;
+    hipError_t e = rows_r2c(g, a, src.kind, out, s, tw);
+    if (e != hipSuccess) return e;
+    if (g.sy > 1) {
+        e = cols(g, g.sy, -1, false, 0, out, out, g.nc1, g.nc1, (long)g.nc1 * g.sy, g.sz, s, tw);
+        if (e != hipSuccess) return e;
+    }
+    if (g.sz > 1) {
+        const long plane = (long)g.nc1 * g.sy;
+        e = cols(g, g.sz, -1, false, 0, out, out, plane, plane, 0, 1, s, tw);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t c2r(const Geom &g, const float2 *in, int mask_p, float scale, float *out, float2 *scratch,
+               hipStream_t s)
+{
+    const float2 *tw = twiddles(s);
+    if (!tw) return hipErrorOutOfMemory;
+    hipError_t e;
+    const long plane = (long)g.nc1 * g.sy;
+    const float2 *cur = in;
+    bool masked = false;
+    if (g.sz > 1) {
+        e = cols(g, g.sz, +1, mask_p > 0, mask_p, cur, scratch, plane, plane, 0, 1, s, tw);
+        if (e != hipSuccess) return e;
+        cur = scratch;
+        masked = true;
+    }
+    if (g.sy > 1) {
+        const bool m = !masked && mask_p > 0;
+        e = cols(g, g.sy, +1, m, mask_p, cur, scratch, g.nc1, g.nc1, plane, g.sz, s, tw);
+        if (e != hipSuccess) return e;
+        cur = scratch;
+        masked = masked || m;
+    }
+    return rows_c2r(g, cur, out, scale, masked ? 0 : mask_p, s, tw);
+}
+
+// ------------------------------------------------------------ update
+
+__device__ __forceinline__ float ipow(float x, int n)
+{
+    float r = 1.0f;
+    for (int i = 0; i < n; i++) r *= x;
+    return r;
+}
+
+// Semi-implicit Euler update of one field on the half spectrum:
+//   dynamic:    s = (cur + dt*sum_t T_t + P_eta*eta) / D(q)          (setDynamic_k)
+//   constraint: s = (sum_t T_t [+ dt^-1/2 P_eta eta]) / L(q), q != 0  (setNotDynamic_k)
+// with T_t = i^m M_t(q) X_t.  Each prefactor of M_t is kept only if its
+// image under the real-part projection survives: (m + sum of its q_d
+// powers over the axes where this bin is not a Nyquist index) even.
+__global__ __launch_bounds__(256) void k_update(UpdateArgs a)
+{
+    const Geom &g = a.g;
+    const long h = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= g.nh) return;
+    const int kx = (int)(h % g.nc1);
+    const long r = h / g.nc1;
+    const int ky = (int)(r % g.sy);
+    const int kz = (int)(r / g.sy);
+    const float qx = (float)(kx < (g.sx + 1) / 2 ? kx : kx - g.sx) * g.sqx;
+    const float qy = (float)(ky < (g.sy + 1) / 2 ? ky : ky - g.sy) * g.sqy;
+    const float qz = (float)(kz < (g.sz + 1) / 2 ? kz : kz - g.sz) * g.sqz;
+    const float q2 = qx * qx + qy * qy + qz * qz;
+    const bool nyx = g.sx > 1 && kx == g.sx / 2;
+    const bool nyy = g.sy > 1 && ky == g.sy / 2;
+    const bool nyz = g.sz > 1 && kz == g.sz / 2;
+    const float iq = h > 0 ? 1.0f / sqrtf(q2) : 0.0f;
+
+    float2 acc = make_float2(0.f, 0.f);
+    for (int t = 0; t < a.nterms; t++) {
+        const TermRefDev &T = a.terms[t];
+        float m = 0.0f;
+        for (int p = T.pres0; p < T.pres0 + T.npres; p++) {
+            const PresDev P = a.pres[p];
+            const int par = T.mult_i + (nyx ? 0 : P.a) + (nyy ? 0 : P.b) + (nyz ? 0 : P.cz);
+            if (par & 1) continue;
+            float v = P.c;
+            if (P.q2n > 0) v *= ipow(q2, P.q2n);
+            if (P.a > 0) v *= ipow(qx, P.a);
+            if (P.b > 0) v *= ipow(qy, P.b);
+            if (P.cz > 0) v *= ipow(qz, P.cz);
+            if (P.invq > 0) v *= ipow(iq, P.invq);
+            m += v;
+        }
+        float2 X;
+        if (T.kind == TERM_CONST)
+            X = h == 0 ? make_float2((float)g.n, 0.f) : make_float2(0.f, 0.f);
+        else
+            X = T.X[h];
+        float2 v = cscale(X, m);
+        if (T.mult_i) v = make_float2(-v.y, v.x);
+        acc = cadd(acc, v);
+    }
+
+    float pn = 0.0f;
+    if (a.noisy) {
+        pn = a.noise_base;
+        if (a.noise_q2n != 0) pn *= sqrtf(ipow(q2, a.noise_q2n));
+        if (a.noise_invq != 0) pn *= sqrtf(ipow(iq, a.noise_invq));
+    }
+
+    float2 s;
+    if (a.dynamic) {
+        s = a.cur[h];
+        s.x += a.dt * acc.x;
+        s.y += a.dt * acc.y;
+        if (a.noisy) {
+            const float2 n = a.noise[h];
+            s.x += pn * n.x;
+            s.y += pn * n.y;
+        }
+        if (a.nimp > 0) {
+            // precalculateImplicit (field_init.cpp:246-267); 1/|q| is zero on the
+            // whole i = j = 0 line
+            const float iqd = (kx > 0 || ky > 0) ? 1.0f / sqrtf(q2) : 0.0f;
+            float D = 1.0f;
+            for (int i = 0; i < a.nimp; i++) {
+                const PresDev P = a.imp[i];
+                float th = P.c;
+                if (P.q2n != 0) th *= ipow(q2, P.q2n);
+                if (P.invq != 0) th *= ipow(iqd, P.invq);
+                D -= a.dt * th;
+            }
+            s.x /= D;
+            s.y /= D;
+        }
+    } else {
+        s = a.nterms > 0 ? acc : a.cur[h];
+        if (a.noisy) {
+            const float2 n = a.noise[h];
+            const float f = a.sdt * pn;
+            if (a.nterms == 0) {
+                s.x = f * n.x;
+                s.y = f * n.y;
+            } else {
+                s.x += f * n.x;
+                s.y += f * n.y;
+            }
+        }
+        if (a.nimp > 0 && h > 0) {
+            float L = 0.0f;
+            for (int i = 0; i < a.nimp; i++) {
+                const PresDev P = a.imp[i];
+                float th = P.c;
+                if (P.q2n != 0) th *= ipow(q2, P.q2n);
+                if (P.invq != 0) th *= ipow(iq, P.invq);
+                L += th;
+            }
+            s.x /= L;
+            s.y /= L;
+        }
+    }
+    a.out[h] = s;
+}
+
+hipError_t update(const UpdateArgs &a, hipStream_t s)
+{
+    const unsigned blocks = (unsigned)((a.g.nh + 255) / 256);
+    hipLaunchKernelGGL(k_update, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace cupss

@@ -1,0 +1,163 @@
+/*
+ * cupss_hip.h -- C-ABI of the MI355X HIP layer of libcupss.
+ *
+ * Plain C types only (pointers, sizes, ints, floats); every entry point
+ * returns 0 (CUPSS_OK) or a negative CUPSS_ERR_* code and never exits; the
+ * message of the last failure on the calling thread is cupss_last_error().
+ *
+ * The reference has no foreign-function interface: its seam between the
+ * host classes and the device is the set of extern "C" launchers in
+ * inc/cupss/field_kernels.cuh:7-19 and inc/cupss/term_kernels.cuh:7-15,
+ * plus the cuFFT/cuRAND/FFTW calls inside field/term (src/field.cpp,
+ * src/field_init.cpp, src/term.cpp, src/term_init.cpp).  This header
+ * replaces that seam at the granularity of a whole system: the C++ API in
+ * include/cupss/ (evolver, field, term, parser) calls these functions, and
+ * so can C, ctypes or any other FFI.  Each declaration names what it
+ * replaces.
+ *
+ * Grids are power-of-two sx in [2, 4096], sy and sz in [1, 4096]; real
+ * arrays are float[sz][sy][sx] (x fastest, idx = k*sx*sy + j*sx + i);
+ * spectra are kept on the device as the Hermitian half
+ * float2[sz][sy][sx/2+1] ("half spectrum", nh bins).
+ */
+#ifndef CUPSS_HIP_H
+#define CUPSS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CUPSS_ABI_VERSION 1
+
+enum {
+    CUPSS_OK = 0,
+    CUPSS_ERR_INVALID = -1,     /* bad argument or id */
+    CUPSS_ERR_UNSUPPORTED = -2, /* size or system outside what the HIP path implements */
+    CUPSS_ERR_STATE = -3,       /* call out of order (e.g. before prepare) */
+    CUPSS_ERR_HIP = -4,         /* a HIP runtime call failed */
+    CUPSS_ERR_NOMEM = -5,       /* device allocation failed */
+    CUPSS_ERR_BUDGET = -6       /* serving: request does not fit the HBM budget */
+};
+
+/* Operator descriptor, same layout as the reference `pres`
+ * (inc/cupss/defines.h:31-39):
+ * preFactor * q^(2*q2n) * (iqx)^iqx (iqy)^iqy (iqz)^iqz * |q|^(-invq). */
+typedef struct cupss_pres {
+    float preFactor;
+    int q2n;
+    int iqx;
+    int iqy;
+    int iqz;
+    int invq;
+} cupss_pres;
+
+typedef struct cupss_system_t *cupss_system;
+
+/* Real-space hook (replaces the call of field::callback on real_array_d /
+ * real_dealiased_d, src/field.cpp:70-86): `real2` is a device float2[N]
+ * array holding the field's real values in .x (.y = 0); `which` is 0 for
+ * the field, 1 for its dealiased copy.  The hook may change .x. */
+typedef void (*cupss_real_hook)(void *user, int field, void *real2, int which);
+/* Fourier-space hook (replaces field::callbackFourier on comp_array_d,
+ * src/field.cpp:48-57): `full2` is a device float2[N] array holding the
+ * full C2C spectrum; the hook may change it (the change is then projected
+ * onto real fields, as the reference's IFFT/normalize/FFT does). */
+typedef void (*cupss_spec_hook)(void *user, int field, void *full2);
+
+const char *cupss_last_error(void);
+int cupss_abi_version(void);
+
+/* evolver constructors + field/term geometry (src/evolver.cpp:13-74,
+ * src/field_init.cpp:191-213): q_d = n*2*PI/(d*s), PI = 3.1415926535f. */
+int cupss_system_create(int sx, int sy, int sz, float dx, float dy, float dz, float dt, cupss_system *out);
+/* evolver::~evolver / field::~field / term::~term (frees every buffer) */
+int cupss_system_destroy(cupss_system s);
+
+/* evolver::createField -> field::common_constructor device allocations
+ * (src/evolver.cpp:177-196, src/field_init.cpp:15-154). */
+int cupss_field_create(cupss_system s, int dynamic, int *field_id);
+/* field::implicit push (src/parser.cpp:728-738) */
+int cupss_field_add_implicit(cupss_system s, int field, cupss_pres p);
+/* field::updateParameter -> precalculateImplicit (src/field.cpp:422-431) */
+int cupss_field_set_implicits(cupss_system s, int field, const cupss_pres *p, int n);
+/* evolver::createTerm (src/evolver.cpp:326-362): a product of `nfactors`
+ * fields (0 = constant term) with summed prefactors. */
+int cupss_term_create(cupss_system s, int field, const int *factors, int nfactors, const cupss_pres *pres,
+                      int npres, int *term_id);
+/* term::precomputePrefactors after a parameter change (src/field.cpp:432-439) */
+int cupss_term_set_prefactors(cupss_system s, int term, const cupss_pres *pres, int npres);
+/* evolver::addNoise (src/evolver.cpp:162-171) + the cuRAND generator of
+ * field_init.cpp:130-137: Gaussian white noise from in-kernel Philox4x32-10
+ * keyed by `seed`, counted by (cell, step, field). */
+int cupss_field_set_noise(cupss_system s, int field, cupss_pres amplitude, uint64_t seed);
+/* field::hasCB/callback and hasCBFourier/callbackFourier (field.h:99-103);
+ * null hooks switch them off. */
+int cupss_field_set_hooks(cupss_system s, int field, cupss_real_hook real_hook, cupss_spec_hook spec_hook,
+                          void *user);
+
+/* field::copyHostToDevice + toComp (src/field.cpp:332-335, 261-273):
+ * `host` holds N floats (the .x of real_array). */
+int cupss_field_set_real(cupss_system s, int field, const float *host);
+/* field::copyRealDeviceToHost (src/field.cpp:346-348): N floats */
+int cupss_field_get_real(cupss_system s, int field, float *host);
+/* real_dealiased (src/field.cpp:77-83): N floats */
+int cupss_field_get_dealiased(cupss_system s, int field, float *host);
+/* comp_array as the half spectrum: 2*nh floats (re, im) */
+int cupss_field_get_spectrum(cupss_system s, int field, float *host);
+int cupss_field_set_spectrum(cupss_system s, int field, const float *host);
+/* number of bins of the half spectrum */
+long cupss_system_half_bins(cupss_system s);
+
+/* prepareProblem's device part (src/evolver.cpp:114-125): dealias orders
+ * (term::prepareDevice, src/term_init.cpp:108-133) and buffers.  Re-callable
+ * after fields are re-initialised; it does not reset the step counter. */
+int cupss_system_prepare(cupss_system s);
+/* dealias order of a field (0 = not dealiased) */
+int cupss_field_aliasing_order(cupss_system s, int field);
+
+/* evolver::advanceTime x nsteps without output (src/evolver.cpp:199-226):
+ * phases A-D, each field's update (setDynamic_k / setNotDynamic_k),
+ * dealias, transforms and noise, all on the system's stream. */
+int cupss_system_advance(cupss_system s, int nsteps);
+/* Same, timed with HIP events recorded on the system's stream around the
+ * launches; writes the elapsed milliseconds. */
+int cupss_system_time_advance(cupss_system s, int nsteps, float *ms);
+int cupss_system_synchronize(cupss_system s);
+int cupss_system_get_step(cupss_system s, uint64_t *step);
+int cupss_system_set_step(cupss_system s, uint64_t step);
+/* use the caller's hipStream_t (null = the library's own stream) */
+int cupss_system_set_stream(cupss_system s, void *hip_stream);
+/* evolver::dt change is not part of the reference API; exposed for sweeps */
+int cupss_system_set_dt(cupss_system s, float dt);
+
+/* Device buffer access for callers that bring their own kernels (e.g. the
+ * reference's user callbacks that launch __global__ functions): device
+ * pointer to the current half spectrum of a field. */
+int cupss_field_spectrum_device(cupss_system s, int field, void **dev_ptr);
+
+/* Stand-alone transforms on caller device buffers (plain pointers), used by
+ * tests and tools: half spectrum out[nh] = R2C(in[N]); real out[N] =
+ * scale * C2R(in) with dealias mask order mask_p (0 = none); `scratch` holds
+ * nh float2 and must not alias `in`. */
+int cupss_r2c(int sx, int sy, int sz, const float *in, void *out, void *hip_stream);
+int cupss_c2r(int sx, int sy, int sz, const void *in, int mask_p, float scale, float *out, void *scratch,
+              void *hip_stream);
+/* Philox normals of a stream (seed, step, stream) for cells [0, n). */
+int cupss_normals(uint64_t seed, uint64_t step, uint32_t stream, float *out, long n, void *hip_stream);
+
+/* Device memory helpers so that callers without a HIP toolchain (ctypes)
+ * can stage data. */
+int cupss_device_alloc(size_t bytes, void **dev_ptr);
+int cupss_device_free(void *dev_ptr);
+int cupss_memcpy_h2d(void *dev_dst, const void *host_src, size_t bytes);
+int cupss_memcpy_d2h(void *host_dst, const void *dev_src, size_t bytes);
+int cupss_device_synchronize(void);
+int cupss_device_count(int *n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
