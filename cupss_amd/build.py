@@ -42,6 +42,40 @@ def _compile(src):
     return obj
 
 
+# C++ programs linked against libcupss.so: the test drivers of tests/cpp and
+# the example solvers of examples/ (binaries land in a bin/ folder beside them)
+TOOLS = [os.path.join(ROOT, "tests", "cpp", "reference_cases.cpp"),
+         os.path.join(ROOT, "tests", "cpp", "system_driver.cpp")]
+
+
+def _tool_sources():
+    out = list(TOOLS)
+    ex = os.path.join(ROOT, "examples")
+    if os.path.isdir(ex):
+        for f in sorted(os.listdir(ex)):
+            if f.endswith(".cpp"):
+                out.append(os.path.join(ex, f))
+    return out
+
+
+def _build_tool(src):
+    bindir = os.path.join(os.path.dirname(src), "bin")
+    os.makedirs(bindir, exist_ok=True)
+    exe = os.path.join(bindir, os.path.splitext(os.path.basename(src))[0])
+    rpath = os.path.relpath(HERE, bindir)
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), src,
+           "-o", exe, "-L" + HERE, "-lcupss", "-Wl,-rpath,$ORIGIN/" + rpath]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("tool build failed: %s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))
+    return exe
+
+
+def build_tools():
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
+        return list(ex.map(_build_tool, _tool_sources()))
+
+
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     srcs = sources()
@@ -53,8 +87,11 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    tools = build_tools()
     if verbose:
         print("built", LIB)
+        for t in tools:
+            print("built", t)
     return LIB
 
 

@@ -104,7 +104,7 @@ field::field(int _sx, int _sy, int _sz, float _dx, float _dy, float _dz)
 
 static cupss_system_t *sys_of(field *f)
 {
-    if (!f->system_p || f->device_id < 0) {
+    if (!f->system_p || f->deviceId() < 0) {
         std::cout << "field " << f->name << " is not attached to an evolver" << std::endl;
         std::exit(1);
     }

@@ -139,6 +139,9 @@ class field {
     int addImplicitString(const std::string &);
     void printImplicitString();
     int updateParameter(const std::string &, float);
+
+    // Extension of this build: id of the field in the device system.
+    int deviceId() const { return device_id; }
 };
 
 #endif // FIELD_H
