@@ -94,6 +94,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="ch2d", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--path", default="auto", choices=["auto", "general"],
+                    help="auto: fused two-kernel step when the system qualifies; general: step program")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build runs on a single GPU (--gpus 1); "
@@ -106,7 +108,10 @@ def main():
     n = dims[0] * dims[1] * dims[2]
     x0 = np.random.default_rng(1).uniform(-0.1, 0.1, n).astype(np.float32)
     s, phi = wl["build"](C.System, C.pres, dims, x0)
-    print("bench: %s, warmup %d, steps %d" % (wl["name"], args.warmup, args.steps), file=sys.stderr, flush=True)
+    s.set_path(0 if args.path == "auto" else 1)
+    path = "fused two-kernel step" if (args.path == "auto" and s.fused_eligible) else "general step program"
+    print("bench: %s, %s, warmup %d, steps %d" % (wl["name"], path, args.warmup, args.steps), file=sys.stderr,
+          flush=True)
     s.step(args.warmup)
     s.synchronize()
     ms = s.time_steps(args.steps)
@@ -127,7 +132,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": wl["name"], "grid": list(dims), "dt": 0.01, "path": "plain per-solve step program"},
+        "config": {"workload": wl["name"], "grid": list(dims), "dt": 0.01, "path": path},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "per": "whole step (all kernels of one advanceTime), 8 B/point algorithmic"},

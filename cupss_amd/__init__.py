@@ -65,6 +65,8 @@ _SIGS = {
     "cupss_system_set_step": (_I, [_VP, ctypes.c_uint64]),
     "cupss_system_set_stream": (_I, [_VP, _VP]),
     "cupss_system_set_dt": (_I, [_VP, _F]),
+    "cupss_system_set_path": (_I, [_VP, _I]),
+    "cupss_system_fused_eligible": (_I, [_VP]),
     "cupss_field_spectrum_device": (_I, [_VP, _I, ctypes.POINTER(_VP)]),
     "cupss_r2c": (_I, [_I, _I, _I, _VP, _VP, _VP]),
     "cupss_c2r": (_I, [_I, _I, _I, _VP, _I, _F, _VP, _VP, _VP]),
@@ -220,6 +222,14 @@ class System:
         out = np.empty(2 * self.nh, np.float32)
         check(self.L.cupss_field_get_spectrum(self.h, f, _fptr(out)))
         return out[0::2] + 1j * out[1::2]
+
+    def set_path(self, mode):
+        """0 = fused two-kernel step when eligible, 1 = general step program."""
+        check(self.L.cupss_system_set_path(self.h, int(mode)))
+
+    @property
+    def fused_eligible(self):
+        return bool(self.L.cupss_system_fused_eligible(self.h))
 
     def aliasing_order(self, f):
         return self.L.cupss_field_aliasing_order(self.h, f)

@@ -129,8 +129,23 @@ __host__ __device__ constexpr int lds_len(int n) { return n + (n >> 5) + 1; }
 
 constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
 
-// elements per thread for a line of n points
+// elements per thread for a line of n points in a column (strided) pass
 __host__ __device__ constexpr int elems_for(int n) { return n >= 256 ? 16 : (n >= 16 ? 8 : n); }
+
+// elements per thread in a row pass: one wavefront (64 lanes) per row from
+// 128 complex points up, 16 elements per lane at most
+__host__ __device__ constexpr int row_elems(int nc)
+{
+    return nc >= 1024 ? 16 : (nc >= 128 ? nc / 64 : (nc >= 4 ? 4 : nc));
+}
+
+// XCD-aware workgroup placement: the hardware deals workgroups to the 8
+// XCDs round-robin, so consecutive linear ids would put neighbouring tiles
+// (which share cache lines) on different L2s.  Linear id b is remapped to
+// work item (b % 8) * (nblk / 8) + b / 8, so each XCD sweeps one contiguous
+// range of tiles.  nblk is a multiple of 8; items >= total are idle.
+__host__ __device__ constexpr long xcd_blocks(long total) { return (total + 7) / 8 * 8; }
+__device__ __forceinline__ long xcd_item(long b, long nblk) { return (b & 7) * (nblk >> 3) + (b >> 3); }
 
 // exp(DIR*2*pi*i*m/L) from the table (L | TW_N, 0 <= m)
 template <int DIR>

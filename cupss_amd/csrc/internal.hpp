@@ -35,6 +35,7 @@ enum TermKind : int {
     TERM_BUFFER = 0,  // X = R2C(product) in a half-spectrum buffer
     TERM_CONST = 1,   // empty product: X = N at q = 0 (term_kernels.cu:58-63)
     TERM_FIELD = 2,   // single field: X = that field's spectrum (copyComp, term.cpp:104-126)
+    TERM_GROUPED = 3, // product evaluated by the fused step as part of a group
 };
 
 struct TermRefDev {

@@ -21,6 +21,7 @@
 #include "fft_core.hpp"
 #include "internal.hpp"
 #include "philox.hpp"
+#include "update_math.hpp"
 
 namespace cupss {
 
@@ -83,11 +84,12 @@ __device__ __forceinline__ float2 load_src(const SrcArgs &a, long row, int m, in
 
 // ------------------------------------------------------------ rows R2C
 
+// rows per workgroup: one wavefront at least
 template <int NC>
 struct RowCfg {
-    static constexpr int E = elems_for(NC);
+    static constexpr int E = row_elems(NC);
     static constexpr int T = NC / E;
-    static constexpr int LPB = T >= 256 ? 1 : 256 / T;
+    static constexpr int LPB = T >= 64 ? 1 : 64 / T;
     static constexpr int THREADS = LPB * T;
     static constexpr int LL = lds_len(NC);
 };
@@ -175,56 +177,50 @@ __global__ __launch_bounds__(256) void k_rows_c2r(const float2 *__restrict__ in,
 
 // ------------------------------------------------------------ columns
 
+// Column tile: TWD adjacent lines of one workgroup (one line per TWD-th
+// thread, so that a row segment of TWD values is read with consecutive
+// lanes).  Valid tiles keep 64 <= TWD*T <= 512 threads.
 template <int N>
 struct ColCfg {
     static constexpr int E = elems_for(N);
     static constexpr int T = N / E;
-    static constexpr int TWD = T >= 64 ? std::max(1, 512 / T) : std::min(64, std::max(1, 256 / T));
-    static constexpr int THREADS = TWD * T;
     static constexpr int LL = lds_len(N);
+    static constexpr bool valid(int twd) { return twd * T >= 64 && twd * T <= 512; }
 };
 
 // C2C along a strided axis.  Line `col` of batch `b` is
 //   p[b*bstride + col + y*stride], y = 0..N-1, col = 0..ncols-1.
-// MASK: dealias mask of order p on the half spectrum (first inverse pass);
-// `axis` tells which axis the lines run along (1 = y in 2D, 2 = z in 3D).
-template <int N, int DIR, bool MASK>
+// Workgroups are placed XCD-aware over the (batch, tile) items.
+// MASK: dealias mask of order p on the half spectrum (first inverse pass).
+template <int N, int TWD, int DIR, bool MASK>
 __global__ __launch_bounds__(512) void k_cols(const float2 *__restrict__ in, float2 *__restrict__ out, long ncols,
-                                              long stride, long bstride, Geom g, int p,
+                                              long stride, long bstride, long nbatch, Geom g, int p,
                                               const float2 *__restrict__ tw)
 {
     using C = ColCfg<N>;
     constexpr int E = C::E, T = C::T;
-    __shared__ float2 lds[C::TWD * C::LL];
-    const int c = threadIdx.x % C::TWD, t = threadIdx.x / C::TWD;
-    const long col = (long)blockIdx.x * C::TWD + c;
-    const long b = blockIdx.y;
+    __shared__ float2 lds[TWD * C::LL];
+    const long ntiles = (ncols + TWD - 1) / TWD;
+    const long item = xcd_item(blockIdx.x, gridDim.x);
+    if (item >= ntiles * nbatch) return;   // whole workgroup idle
+    const long b = item / ntiles;
+    const long tile = item - b * ntiles;
+    const int c = threadIdx.x % TWD, t = threadIdx.x / TWD;
+    const long col = tile * TWD + c;
     const bool active = col < ncols;
     const float2 *src = in + b * bstride + col;
     float2 v[E];
 #pragma unroll
     for (int e = 0; e < E; e++) v[e] = active ? src[(long)(t + e * T) * stride] : make_float2(0.f, 0.f);
     if constexpr (MASK) {
-        // bin indices of this line: 2D lines run along y (col = kx);
-        // 3D first pass lines run along z (col = ky*nc1 + kx)
+        // bin indices: 2D lines run along y (col = kx); the 3D first inverse
+        // pass runs along z (col = ky*nc1 + kx)
         const int kx = (int)(col % g.nc1);
         const int ky0 = (int)(col / g.nc1);
-        const int p1 = p + 1;
-        const bool okx = kx <= g.sx / p1;
 #pragma unroll
         for (int e = 0; e < E; e++) {
             const int y = t + e * T;
-            int nyv, nzv;
-            if (g.sz > 1) {
-                nyv = ky0;
-                nzv = y;
-            } else {
-                nyv = y;
-                nzv = 0;
-            }
-            int ny = nyv > g.sy / 2 ? nyv - g.sy : nyv;
-            int nz = nzv > g.sz / 2 ? nzv - g.sz : nzv;
-            const bool keep = okx && abs(ny) <= g.sy / p1 && abs(nz) <= g.sz / p1;
+            const bool keep = g.sz > 1 ? keep_mode(g, p, kx, ky0, y) : keep_mode(g, p, kx, y, 0);
             if (!keep) v[e] = make_float2(0.f, 0.f);
         }
     }
@@ -287,33 +283,65 @@ static hipError_t rows_c2r(const Geom &g, const float2 *in, float *out, float sc
     }
 }
 
+template <int N, int TWD>
+static void launch_cols(int dir, bool mask, int p, const float2 *in, float2 *out, long ncols, long stride,
+                        long bstride, long nbatch, const Geom &g, hipStream_t s, const float2 *tw)
+{
+    const long total = (ncols + TWD - 1) / TWD * nbatch;
+    dim3 grid((unsigned)xcd_blocks(total)), block(TWD * ColCfg<N>::T);
+    if (dir < 0)
+        hipLaunchKernelGGL((k_cols<N, TWD, -1, false>), grid, block, 0, s, in, out, ncols, stride, bstride, nbatch,
+                           g, p, tw);
+    else if (mask)
+        hipLaunchKernelGGL((k_cols<N, TWD, 1, true>), grid, block, 0, s, in, out, ncols, stride, bstride, nbatch, g,
+                           p, tw);
+    else
+        hipLaunchKernelGGL((k_cols<N, TWD, 1, false>), grid, block, 0, s, in, out, ncols, stride, bstride, nbatch,
+                           g, p, tw);
+}
+
+// widest valid tile that still gives >= 1024 workgroups, else the narrowest
+template <int N>
+static void cols_n(int dir, bool mask, int p, const float2 *in, float2 *out, long ncols, long stride, long bstride,
+                   long nbatch, const Geom &g, hipStream_t s, const float2 *tw)
+{
+    constexpr int widths[5] = {64, 16, 4, 2, 1};
+    int pick = 0;
+    for (int w : widths) {
+        if (!ColCfg<N>::valid(w)) continue;
+        pick = w;
+        if ((ncols + w - 1) / w * nbatch >= 1024) break;
+    }
+    switch (pick) {
+    case 64:
+        if constexpr (ColCfg<N>::valid(64)) launch_cols<N, 64>(dir, mask, p, in, out, ncols, stride, bstride, nbatch, g, s, tw);
+        break;
+    case 16:
+        if constexpr (ColCfg<N>::valid(16)) launch_cols<N, 16>(dir, mask, p, in, out, ncols, stride, bstride, nbatch, g, s, tw);
+        break;
+    case 4:
+        if constexpr (ColCfg<N>::valid(4)) launch_cols<N, 4>(dir, mask, p, in, out, ncols, stride, bstride, nbatch, g, s, tw);
+        break;
+    case 2:
+        if constexpr (ColCfg<N>::valid(2)) launch_cols<N, 2>(dir, mask, p, in, out, ncols, stride, bstride, nbatch, g, s, tw);
+        break;
+    case 1:
+        if constexpr (ColCfg<N>::valid(1)) launch_cols<N, 1>(dir, mask, p, in, out, ncols, stride, bstride, nbatch, g, s, tw);
+        break;
+    }
+}
+
 #define CUPSS_COL_CASES(X) \
     X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
 
 static hipError_t cols(const Geom &g, int n, int dir, bool mask, int p, const float2 *in, float2 *out, long ncols,
                        long stride, long bstride, long nbatch, hipStream_t s, const float2 *tw)
 {
-    if (n == 1) {
-        if (in != out)
-            return hipMemcpyAsync(out, in, sizeof(float2) * (size_t)ncols * nbatch, hipMemcpyDeviceToDevice, s);
-        return hipSuccess;
-    }
     switch (n) {
-#define X(N)                                                                                                   \
-    case N: {                                                                                                  \
-        dim3 grid((unsigned)((ncols + ColCfg<N>::TWD - 1) / ColCfg<N>::TWD), (unsigned)nbatch);               \
-        dim3 block(ColCfg<N>::THREADS);                                                                        \
-        if (dir < 0)                                                                                           \
-            hipLaunchKernelGGL((k_cols<N, -1, false>), grid, block, 0, s, in, out, ncols, stride, bstride, g, p, \
-                               tw);                                                                            \
-        else if (mask)                                                                                         \
-            hipLaunchKernelGGL((k_cols<N, 1, true>), grid, block, 0, s, in, out, ncols, stride, bstride, g, p,  \
-                               tw);                                                                            \
-        else                                                                                                   \
-            hipLaunchKernelGGL((k_cols<N, 1, false>), grid, block, 0, s, in, out, ncols, stride, bstride, g, p, \
-                               tw);                                                                            \
-        return hipGetLastError();                                                                              \
-    }
+#define X(N)                                                                              \
+    case N:                                                                               \
+        cols_n<N>(dir, mask, p, in, out, ncols, stride, bstride, nbatch, g, s, tw);      \
+        return hipGetLastError();
         CUPSS_COL_CASES(X)
 #undef X
     default:
@@ -374,128 +402,30 @@ hipError_t c2r(const Geom &g, const float2 *in, int mask_p, float scale, float *
 
 // ------------------------------------------------------------ update
 
-__device__ __forceinline__ float ipow(float x, int n)
-{
-    float r = 1.0f;
-    for (int i = 0; i < n; i++) r *= x;
-    return r;
-}
-
-// Semi-implicit Euler update of one field on the half spectrum:
+// Semi-implicit Euler update of one field on the half spectrum (one
+// workgroup per spectral row, lanes over kx):
 //   dynamic:    s = (cur + dt*sum_t T_t + P_eta*eta) / D(q)          (setDynamic_k)
 //   constraint: s = (sum_t T_t [+ dt^-1/2 P_eta eta]) / L(q), q != 0  (setNotDynamic_k)
-// with T_t = i^m M_t(q) X_t.  Each prefactor of M_t is kept only if its
-// image under the real-part projection survives: (m + sum of its q_d
-// powers over the axes where this bin is not a Nyquist index) even.
 __global__ __launch_bounds__(256) void k_update(UpdateArgs a)
 {
     const Geom &g = a.g;
-    const long h = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= g.nh) return;
-    const int kx = (int)(h % g.nc1);
-    const long r = h / g.nc1;
-    const int ky = (int)(r % g.sy);
-    const int kz = (int)(r / g.sy);
-    const float qx = (float)(kx < (g.sx + 1) / 2 ? kx : kx - g.sx) * g.sqx;
-    const float qy = (float)(ky < (g.sy + 1) / 2 ? ky : ky - g.sy) * g.sqy;
-    const float qz = (float)(kz < (g.sz + 1) / 2 ? kz : kz - g.sz) * g.sqz;
-    const float q2 = qx * qx + qy * qy + qz * qz;
-    const bool nyx = g.sx > 1 && kx == g.sx / 2;
-    const bool nyy = g.sy > 1 && ky == g.sy / 2;
-    const bool nyz = g.sz > 1 && kz == g.sz / 2;
-    const float iq = h > 0 ? 1.0f / sqrtf(q2) : 0.0f;
-
-    float2 acc = make_float2(0.f, 0.f);
-    for (int t = 0; t < a.nterms; t++) {
-        const TermRefDev &T = a.terms[t];
-        float m = 0.0f;
-        for (int p = T.pres0; p < T.pres0 + T.npres; p++) {
-            const PresDev P = a.pres[p];
-            const int par = T.mult_i + (nyx ? 0 : P.a) + (nyy ? 0 : P.b) + (nyz ? 0 : P.cz);
-            if (par & 1) continue;
-            float v = P.c;
-            if (P.q2n > 0) v *= ipow(q2, P.q2n);
-            if (P.a > 0) v *= ipow(qx, P.a);
-            if (P.b > 0) v *= ipow(qy, P.b);
-            if (P.cz > 0) v *= ipow(qz, P.cz);
-            if (P.invq > 0) v *= ipow(iq, P.invq);
-            m += v;
-        }
-        float2 X;
-        if (T.kind == TERM_CONST)
-            X = h == 0 ? make_float2((float)g.n, 0.f) : make_float2(0.f, 0.f);
-        else
-            X = T.X[h];
-        float2 v = cscale(X, m);
-        if (T.mult_i) v = make_float2(-v.y, v.x);
-        acc = cadd(acc, v);
+    const long row = blockIdx.x;
+    const int ky = (int)(row % g.sy);
+    const int kz = (int)(row / g.sy);
+    for (int kx = threadIdx.x; kx < g.nc1; kx += blockDim.x) {
+        const Bin b = make_bin(g, kx, ky, kz);
+        float2 acc = make_float2(0.f, 0.f);
+        for (int t = 0; t < a.nterms; t++) acc = cadd(acc, term_value(a, t, b, g));
+        const float2 eta = a.noisy ? a.noise[b.h] : make_float2(0.f, 0.f);
+        a.out[b.h] = finish_update(a, b, a.cur[b.h], acc, eta);
     }
-
-    float pn = 0.0f;
-    if (a.noisy) {
-        pn = a.noise_base;
-        if (a.noise_q2n != 0) pn *= sqrtf(ipow(q2, a.noise_q2n));
-        if (a.noise_invq != 0) pn *= sqrtf(ipow(iq, a.noise_invq));
-    }
-
-    float2 s;
-    if (a.dynamic) {
-        s = a.cur[h];
-        s.x += a.dt * acc.x;
-        s.y += a.dt * acc.y;
-        if (a.noisy) {
-            const float2 n = a.noise[h];
-            s.x += pn * n.x;
-            s.y += pn * n.y;
-        }
-        if (a.nimp > 0) {
-            // precalculateImplicit (field_init.cpp:246-267); 1/|q| is zero on the
-            // whole i = j = 0 line
-            const float iqd = (kx > 0 || ky > 0) ? 1.0f / sqrtf(q2) : 0.0f;
-            float D = 1.0f;
-            for (int i = 0; i < a.nimp; i++) {
-                const PresDev P = a.imp[i];
-                float th = P.c;
-                if (P.q2n != 0) th *= ipow(q2, P.q2n);
-                if (P.invq != 0) th *= ipow(iqd, P.invq);
-                D -= a.dt * th;
-            }
-            s.x /= D;
-            s.y /= D;
-        }
-    } else {
-        s = a.nterms > 0 ? acc : a.cur[h];
-        if (a.noisy) {
-            const float2 n = a.noise[h];
-            const float f = a.sdt * pn;
-            if (a.nterms == 0) {
-                s.x = f * n.x;
-                s.y = f * n.y;
-            } else {
-                s.x += f * n.x;
-                s.y += f * n.y;
-            }
-        }
-        if (a.nimp > 0 && h > 0) {
-            float L = 0.0f;
-            for (int i = 0; i < a.nimp; i++) {
-                const PresDev P = a.imp[i];
-                float th = P.c;
-                if (P.q2n != 0) th *= ipow(q2, P.q2n);
-                if (P.invq != 0) th *= ipow(iq, P.invq);
-                L += th;
-            }
-            s.x /= L;
-            s.y /= L;
-        }
-    }
-    a.out[h] = s;
 }
 
 hipError_t update(const UpdateArgs &a, hipStream_t s)
 {
-    const unsigned blocks = (unsigned)((a.g.nh + 255) / 256);
-    hipLaunchKernelGGL(k_update, dim3(blocks), dim3(256), 0, s, a);
+    const unsigned rows = (unsigned)((long)a.g.sy * a.g.sz);
+    const unsigned threads = a.g.nc1 >= 256 ? 256 : (a.g.nc1 >= 128 ? 128 : 64);
+    hipLaunchKernelGGL(k_update, dim3(rows), dim3(threads), 0, s, a);
     return hipGetLastError();
 }
 

@@ -19,10 +19,12 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
 
+#include "fused.hpp"
 #include "internal.hpp"
 #include "philox.hpp"
 
@@ -54,6 +56,11 @@ struct FieldRec {
     float2 *spec[2] = {nullptr, nullptr};
     int cur = 0;
     float *dreal = nullptr;
+    // dealiased real copy: 0 = zero (never updated, the reference's step-0
+    // state), 1 = dreal holds C2R(mask*spec), 2 = same but not yet written
+    // to dreal (after fused steps), 3 = stale (spec reset by set_real after
+    // updates; the reference keeps the old real_dealiased then)
+    int dstate = 0;
     float2 *eta = nullptr;
     cupss_real_hook real_hook = nullptr;
     cupss_spec_hook spec_hook = nullptr;
@@ -78,6 +85,17 @@ struct cupss_system_t {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     bool prepared = false;
+    int path_mode = 0;          // 0 = fused step when eligible, 1 = general step only
+    struct Plan {
+        bool ok = false;
+        int phi = -1;
+        std::vector<int> real;                 // fields read in real space by the products
+        std::vector<std::vector<int>> groups;  // phi's product terms with equal prefactors
+        std::vector<int> derived;              // constraint fields psi = M_psi phi
+    } plan;
+    float2 *fz[FUSED_MAX_REAL] = {nullptr, nullptr, nullptr};
+    float2 *fx[FUSED_MAX_GROUPS] = {nullptr, nullptr, nullptr, nullptr};
+    long fused_steps = 0;       // steps taken by the fused path (diagnostics)
     uint64_t step = 0;
     int device = 0;
 };
@@ -156,6 +174,8 @@ extern "C" int cupss_system_destroy(cupss_system s)
         hipFree(F.eta);
     }
     for (auto &T : s->t) hipFree(T.xhat);
+    for (auto *p : s->fz) hipFree(p);
+    for (auto *p : s->fx) hipFree(p);
     hipFree(s->scratch);
     hipFree(s->rtmp);
     hipFree(s->full);
@@ -258,10 +278,18 @@ extern "C" int cupss_field_set_hooks(cupss_system s, int field, cupss_real_hook 
     return CUPSS_OK;
 }
 
+static int materialize(cupss_system s, int fi);
+static void plan_fused(cupss_system s);
+
 extern "C" int cupss_field_set_real(cupss_system s, int field, const float *host)
 {
     if (int r = check_field(s, field)) return r;
     if (!host) return fail(CUPSS_ERR_INVALID, "null host pointer");
+    // the dealiased copy keeps its old values (prepareProblem re-runs only
+    // toComp, src/evolver.cpp:114-117): write them out before the spectrum
+    // changes, and mark them stale
+    if (int r = materialize(s, field)) return r;
+    if (s->f[field].dstate == 1) s->f[field].dstate = 3;
     HIP_TRY(hipMemcpyAsync(s->rtmp, host, sizeof(float) * s->g.n, hipMemcpyHostToDevice, s->stream));
     R2CSource src{};
     src.kind = LOAD_REAL;
@@ -293,6 +321,7 @@ extern "C" int cupss_field_get_dealiased(cupss_system s, int field, float *host)
         std::memset(host, 0, sizeof(float) * s->g.n);
         return CUPSS_OK;
     }
+    if (int r = materialize(s, field)) return r;
     HIP_TRY(hipMemcpyAsync(host, F.dreal, sizeof(float) * s->g.n, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     return CUPSS_OK;
@@ -360,6 +389,7 @@ extern "C" int cupss_system_prepare(cupss_system s)
             if (hipMalloc(&T.xhat, sizeof(float2) * s->g.nh) != hipSuccess)
                 return fail(CUPSS_ERR_NOMEM, "device allocation failed");
     HIP_TRY(hipStreamSynchronize(s->stream));
+    plan_fused(s);
     s->prepared = true;
     return CUPSS_OK;
 }
@@ -526,9 +556,23 @@ static int post_update(cupss_system s, int fi)
         src.f[0] = s->rtmp;
         src.nf = 1;
         HIP_TRY(r2c(g, src, spec, s->stream));
+        if (F.order > 0) F.dstate = 1;
         return CUPSS_OK;
     }
-    if (F.order > 0) HIP_TRY(c2r(g, spec, F.order, norm, F.dreal, s->scratch, s->stream));
+    if (F.order > 0) {
+        HIP_TRY(c2r(g, spec, F.order, norm, F.dreal, s->scratch, s->stream));
+        F.dstate = 1;
+    }
+    return CUPSS_OK;
+}
+
+// write the dealiased copy of a field whose copy is current but not stored
+static int materialize(cupss_system s, int fi)
+{
+    FieldRec &F = s->f[fi];
+    if (F.dstate != 2 || !F.dreal || F.order <= 0) return CUPSS_OK;
+    HIP_TRY(c2r(s->g, F.spec[F.cur], F.order, 1.0f / (float)s->g.n, F.dreal, s->scratch, s->stream));
+    F.dstate = 1;
     return CUPSS_OK;
 }
 
@@ -578,14 +622,240 @@ static int run_phase(cupss_system s, int dynamic)
     return CUPSS_OK;
 }
 
-extern "C" int cupss_system_advance(cupss_system s, int nsteps)
+static int general_steps(cupss_system s, int nsteps)
 {
-    if (int r = check_sys(s)) return r;
-    if (!s->prepared) return fail(CUPSS_ERR_STATE, "cupss_system_prepare must be called before advancing");
+    for (int i = 0; i < (int)s->f.size(); i++)
+        if (int r = materialize(s, i)) return r;
     for (int n = 0; n < nsteps; n++) {
         if (int r = run_phase(s, 0)) return r;
         if (int r = run_phase(s, 1)) return r;
         s->step += 1;
+    }
+    return CUPSS_OK;
+}
+
+static bool hooks_present(cupss_system s)
+{
+    for (auto &F : s->f)
+        if (F.real_hook || F.spec_hook) return true;
+    return false;
+}
+
+static int fused_steps(cupss_system s, int nsteps);
+
+extern "C" int cupss_system_advance(cupss_system s, int nsteps)
+{
+    if (int r = check_sys(s)) return r;
+    if (!s->prepared) return fail(CUPSS_ERR_STATE, "cupss_system_prepare must be called before advancing");
+    if (nsteps <= 0) return CUPSS_OK;
+    if (s->path_mode == 0 && s->plan.ok && !hooks_present(s)) {
+        // a stale dealiased copy is consumed by one general step first
+        if (s->f[s->plan.phi].dstate == 3) {
+            if (int r = general_steps(s, 1)) return r;
+            nsteps -= 1;
+        }
+        return fused_steps(s, nsteps);
+    }
+    return general_steps(s, nsteps);
+}
+
+extern "C" int cupss_system_set_path(cupss_system s, int mode)
+{
+    if (int r = check_sys(s)) return r;
+    if (mode != 0 && mode != 1) return fail(CUPSS_ERR_INVALID, "path mode is 0 (auto) or 1 (general)");
+    s->path_mode = mode;
+    return CUPSS_OK;
+}
+
+extern "C" int cupss_system_fused_eligible(cupss_system s)
+{
+    if (check_sys(s)) return 0;
+    return s->plan.ok && !hooks_present(s) ? 1 : 0;
+}
+
+// ------------------------------------------------------------ fused path
+
+static bool same_pres(const std::vector<cupss_pres> &a, const std::vector<cupss_pres> &b)
+{
+    if (a.size() != b.size()) return false;
+    for (size_t i = 0; i < a.size(); i++)
+        if (std::memcmp(&a[i], &b[i], sizeof(cupss_pres)) != 0) return false;
+    return true;
+}
+
+// Is the system one dynamic field phi plus derived fields psi = M_psi phi
+// (constraint, one single-field term of phi, no implicit, no noise) that
+// appear only in phi's products?  Then the fused 2D step applies.
+static void plan_fused(cupss_system s)
+{
+    auto &P = s->plan;
+    P = cupss_system_t::Plan();
+    if (!fused_supported(s->g.sx, s->g.sy, s->g.sz)) return;
+    int phi = -1;
+    for (int i = 0; i < (int)s->f.size(); i++)
+        if (s->f[i].dynamic) {
+            if (phi >= 0) return;
+            phi = i;
+        }
+    if (phi < 0) return;
+    std::vector<char> derived(s->f.size(), 0);
+    for (int i = 0; i < (int)s->f.size(); i++) {
+        if (i == phi) continue;
+        const FieldRec &F = s->f[i];
+        if (!F.imp.empty() || F.noisy || F.terms.size() != 1) return;
+        const TermRec &T = s->t[F.terms[0]];
+        if (T.fac.size() != 1 || T.fac[0] != phi) return;
+        if ((int)T.pres.size() > FUSED_MAX_LPRES) return;
+        derived[i] = 1;
+    }
+    // phi's terms
+    std::vector<int> prods;
+    for (int ti : s->f[phi].terms) {
+        const TermRec &T = s->t[ti];
+        if (T.fac.size() == 1) {
+            if (T.fac[0] != phi) return;   // linear terms of phi only
+        } else if (T.fac.size() >= 2) {
+            if ((int)T.fac.size() > FUSED_MAX_FAC) return;
+            for (int c : T.fac) {
+                if (c != phi && !derived[c]) return;
+                if (std::find(P.real.begin(), P.real.end(), c) == P.real.end()) P.real.push_back(c);
+            }
+            prods.push_back(ti);
+        }
+    }
+    if ((int)P.real.size() > FUSED_MAX_REAL || (int)prods.size() > FUSED_MAX_PRODS) return;
+    int lp = 0;
+    for (int f : P.real)
+        if (f != phi) lp += (int)s->t[s->f[f].terms[0]].pres.size();
+    if (lp > FUSED_MAX_LPRES) return;
+    for (int ti : prods) {
+        bool placed = false;
+        for (auto &grp : P.groups)
+            if (same_pres(s->t[grp[0]].pres, s->t[ti].pres)) {
+                grp.push_back(ti);
+                placed = true;
+                break;
+            }
+        if (!placed) P.groups.push_back({ti});
+    }
+    if ((int)P.groups.size() > FUSED_MAX_GROUPS) return;
+    for (int i = 0; i < (int)s->f.size(); i++)
+        if (derived[i]) P.derived.push_back(i);
+    P.phi = phi;
+    P.ok = true;
+}
+
+static int fused_steps(cupss_system s, int nsteps)
+{
+    const auto &P = s->plan;
+    const Geom &g = s->g;
+    FieldRec &F = s->f[P.phi];
+    for (size_t i = 0; i < P.real.size(); i++)
+        if (!s->fz[i] && hipMalloc(&s->fz[i], sizeof(float2) * g.nh) != hipSuccess)
+            return fail(CUPSS_ERR_NOMEM, "device allocation failed");
+    for (size_t i = 0; i < P.groups.size(); i++)
+        if (!s->fx[i] && hipMalloc(&s->fx[i], sizeof(float2) * g.nh) != hipSuccess)
+            return fail(CUPSS_ERR_NOMEM, "device allocation failed");
+
+    // column-pass arguments: phi's update with grouped product terms
+    FusedColArgs ca;
+    std::memset(&ca, 0, sizeof(ca));
+    build_update(s, P.phi, ca.u);
+    ca.u.noise = F.eta;
+    ca.ngroups = (int)P.groups.size();
+    for (size_t gi = 0; gi < P.groups.size(); gi++) {
+        ca.X[gi] = s->fx[gi];
+        for (size_t k = 0; k < F.terms.size(); k++)
+            for (int ti : P.groups[gi])
+                if (F.terms[k] == ti) {
+                    ca.u.terms[k].kind = TERM_GROUPED;
+                    if (ti == P.groups[gi][0]) ca.gterm[gi] = (int)k;
+                }
+    }
+    ca.nreal = (int)P.real.size();
+    int lp = 0;
+    for (size_t i = 0; i < P.real.size(); i++) {
+        const int f = P.real[i];
+        ca.p[i] = s->f[f].order;
+        ca.Z[i] = s->fz[i];
+        ca.zero[i] = (f == P.phi && F.dstate == 0) ? 1 : 0;
+        if (f != P.phi) {
+            const TermRec &T = s->t[s->f[f].terms[0]];
+            const int m = (T.pres[0].iqx + T.pres[0].iqy + T.pres[0].iqz) % 2;
+            ca.lmul[i].mult_i = m;
+            ca.lmul[i].pres0 = lp;
+            ca.lmul[i].npres = (int)T.pres.size();
+            for (const cupss_pres &p : T.pres) {
+                const int iu = p.iqx + p.iqy + p.iqz;
+                PresDev &D = ca.lpres[lp++];
+                D.c = p.preFactor * (float)(-2 * (((iu - m) / 2) % 2) + 1);
+                D.q2n = p.q2n;
+                D.a = p.iqx;
+                D.b = p.iqy;
+                D.cz = p.iqz;
+                D.invq = p.invq;
+            }
+        }
+    }
+    // row-pass arguments
+    FusedRowArgs ra;
+    std::memset(&ra, 0, sizeof(ra));
+    ra.nrows = (long)g.sy * g.sz;
+    ra.scale = 1.0f / (float)g.n;
+    ra.nreal = ca.nreal;
+    for (int i = 0; i < ca.nreal; i++) ra.Z[i] = s->fz[i];
+    ra.ngroups = ca.ngroups;
+    int np = 0;
+    for (size_t gi = 0; gi < P.groups.size(); gi++) {
+        ra.g0[gi] = np;
+        ra.X[gi] = s->fx[gi];
+        for (int ti : P.groups[gi]) {
+            const TermRec &T = s->t[ti];
+            ra.nfac[np] = (int)T.fac.size();
+            for (size_t c = 0; c < T.fac.size(); c++)
+                ra.fac[np][c] = (int)(std::find(P.real.begin(), P.real.end(), T.fac[c]) - P.real.begin());
+            np++;
+        }
+    }
+    ra.g0[P.groups.size()] = np;
+    ra.noisy = F.noisy;
+    ra.eta = F.eta;
+    ra.key0 = (uint32_t)F.seed;
+    ra.key1 = (uint32_t)(F.seed >> 32);
+    ra.stream = (uint32_t)P.phi;
+
+    // prologue: the real fields of the first step from the current state
+    ca.spec_in = F.spec[F.cur];
+    ca.spec_out = F.spec[F.cur ^ 1];
+    ca.do_update = 0;
+    if (ca.nreal > 0) HIP_TRY(fused_cols(ca, g.sy, s->stream));
+    for (int i = 0; i < ca.nreal; i++) ca.zero[i] = 0;
+    ca.do_update = 1;
+    const bool need_rows = ca.nreal > 0 || ra.noisy;
+    for (int n = 0; n < nsteps; n++) {
+        if (need_rows) {
+            ra.step_lo = (uint32_t)s->step;
+            ra.step_hi = (uint32_t)(s->step >> 32);
+            HIP_TRY(fused_rows(ra, g.nc, s->stream));
+        }
+        ca.spec_in = F.spec[F.cur];
+        ca.spec_out = F.spec[F.cur ^ 1];
+        HIP_TRY(fused_cols(ca, g.sy, s->stream));
+        F.cur ^= 1;
+        s->step += 1;
+    }
+    s->fused_steps += nsteps;
+    // state of the general path: phi's dealiased copy is current; each
+    // derived field holds P(M_psi phi) of the last step's starting phi
+    if (F.order > 0) F.dstate = 2;
+    for (int d : P.derived) {
+        FieldRec &D = s->f[d];
+        UpdateArgs a;
+        build_update(s, d, a);
+        a.terms[0].X = F.spec[F.cur ^ 1];
+        a.out = D.spec[D.cur];
+        HIP_TRY(update(a, s->stream));
+        if (D.order > 0) D.dstate = 2;
     }
     return CUPSS_OK;
 }

@@ -1,0 +1,171 @@
+// update_math.hpp -- per-bin spectral arithmetic shared by the update
+// kernel and the fused step kernels.
+//
+//   q grid (src/field_init.cpp:207-213), term multipliers (S2,
+//   src/term_init.cpp:135-200), the projection onto real fields of the
+//   reference's comp->real->comp round trip (src/field.cpp:63-89, see
+//   multiplier()), noise amplitude (src/field_init.cpp:268-278) and the
+//   semi-implicit update of setDynamic_k / setNotDynamic_k
+//   (src/field_kernels.cu:130-227).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fft_core.hpp"
+#include "internal.hpp"
+
+namespace cupss {
+
+struct Bin {
+    long h;          // flat index in the half spectrum
+    int kx, ky, kz;
+    float qx, qy, qz, q2;
+    float iq;        // 1/|q|, 0 at q = 0 (terms, noise, constraint division)
+    float iqd;       // 1/|q|, 0 on the i = j = 0 line (dynamic implicit, Q8)
+    bool nyx, nyy, nyz;
+};
+
+__device__ __forceinline__ float ipow(float x, int n)
+{
+    float r = 1.0f;
+    for (int i = 0; i < n; i++) r *= x;
+    return r;
+}
+
+__device__ __forceinline__ Bin make_bin(const Geom &g, int kx, int ky, int kz)
+{
+    Bin b;
+    b.kx = kx;
+    b.ky = ky;
+    b.kz = kz;
+    b.h = ((long)kz * g.sy + ky) * g.nc1 + kx;
+    b.qx = (float)(kx < (g.sx + 1) / 2 ? kx : kx - g.sx) * g.sqx;
+    b.qy = (float)(ky < (g.sy + 1) / 2 ? ky : ky - g.sy) * g.sqy;
+    b.qz = (float)(kz < (g.sz + 1) / 2 ? kz : kz - g.sz) * g.sqz;
+    b.q2 = b.qx * b.qx + b.qy * b.qy + b.qz * b.qz;
+    b.nyx = g.sx > 1 && kx == g.sx / 2;
+    b.nyy = g.sy > 1 && ky == g.sy / 2;
+    b.nyz = g.sz > 1 && kz == g.sz / 2;
+    const float r = 1.0f / sqrtf(b.q2);
+    b.iq = b.h > 0 ? r : 0.0f;
+    b.iqd = (kx > 0 || ky > 0) ? r : 0.0f;
+    return b;
+}
+
+// M(q) of a term, with the real-part projection folded in: a prefactor
+// monomial i^m c q^2n qx^a qy^b qz^c |q|^-k survives the reference's
+// IFFT/.y=0/FFT round trip iff m + (sum of its q_d powers over the axes
+// where this bin is not a Nyquist index) is even; otherwise its image
+// cancels against its Hermitian partner and it contributes nothing.
+__device__ __forceinline__ float multiplier(const PresDev *__restrict__ pres, int p0, int np, int mult_i,
+                                            const Bin &b)
+{
+    float m = 0.0f;
+    for (int p = p0; p < p0 + np; p++) {
+        const PresDev P = pres[p];
+        const int par = mult_i + (b.nyx ? 0 : P.a) + (b.nyy ? 0 : P.b) + (b.nyz ? 0 : P.cz);
+        if (par & 1) continue;
+        float v = P.c;
+        if (P.q2n > 0) v *= ipow(b.q2, P.q2n);
+        if (P.a > 0) v *= ipow(b.qx, P.a);
+        if (P.b > 0) v *= ipow(b.qy, P.b);
+        if (P.cz > 0) v *= ipow(b.qz, P.cz);
+        if (P.invq > 0) v *= ipow(b.iq, P.invq);
+        m += v;
+    }
+    return m;
+}
+
+// i^mult_i * m * X
+__device__ __forceinline__ float2 apply_mult(float m, int mult_i, float2 X)
+{
+    const float2 v = make_float2(X.x * m, X.y * m);
+    return mult_i ? make_float2(-v.y, v.x) : v;
+}
+
+__device__ __forceinline__ float noise_amplitude(const UpdateArgs &a, const Bin &b)
+{
+    float pn = a.noise_base;
+    if (a.noise_q2n != 0) pn *= sqrtf(ipow(b.q2, a.noise_q2n));
+    if (a.noise_invq != 0) pn *= sqrtf(ipow(b.iq, a.noise_invq));
+    return pn;
+}
+
+// The value of term t at this bin, for TERM_FIELD / TERM_CONST terms and
+// for TERM_BUFFER terms whose transform is read from memory.
+__device__ __forceinline__ float2 term_value(const UpdateArgs &a, int t, const Bin &b, const Geom &g)
+{
+    const TermRefDev &T = a.terms[t];
+    float2 X;
+    if (T.kind == TERM_CONST)
+        X = b.h == 0 ? make_float2((float)g.n, 0.f) : make_float2(0.f, 0.f);
+    else
+        X = T.X[b.h];
+    return apply_mult(multiplier(a.pres, T.pres0, T.npres, T.mult_i, b), T.mult_i, X);
+}
+
+// New spectrum value of the field from its current value, the sum of its
+// explicit terms and its noise transform.
+__device__ __forceinline__ float2 finish_update(const UpdateArgs &a, const Bin &b, float2 cur, float2 acc,
+                                                float2 eta)
+{
+    float2 s;
+    if (a.dynamic) {
+        s = cur;
+        s.x += a.dt * acc.x;
+        s.y += a.dt * acc.y;
+        if (a.noisy) {
+            const float pn = noise_amplitude(a, b);
+            s.x += pn * eta.x;
+            s.y += pn * eta.y;
+        }
+        if (a.nimp > 0) {
+            float D = 1.0f;
+            for (int i = 0; i < a.nimp; i++) {
+                const PresDev P = a.imp[i];
+                float th = P.c;
+                if (P.q2n != 0) th *= ipow(b.q2, P.q2n);
+                if (P.invq != 0) th *= ipow(b.iqd, P.invq);
+                D -= a.dt * th;
+            }
+            s.x /= D;
+            s.y /= D;
+        }
+    } else {
+        s = a.nterms > 0 ? acc : cur;
+        if (a.noisy) {
+            const float f = a.sdt * noise_amplitude(a, b);
+            if (a.nterms == 0) {
+                s.x = f * eta.x;
+                s.y = f * eta.y;
+            } else {
+                s.x += f * eta.x;
+                s.y += f * eta.y;
+            }
+        }
+        if (a.nimp > 0 && b.h > 0) {
+            float L = 0.0f;
+            for (int i = 0; i < a.nimp; i++) {
+                const PresDev P = a.imp[i];
+                float th = P.c;
+                if (P.q2n != 0) th *= ipow(b.q2, P.q2n);
+                if (P.invq != 0) th *= ipow(b.iq, P.invq);
+                L += th;
+            }
+            s.x /= L;
+            s.y /= L;
+        }
+    }
+    return s;
+}
+
+// dealias_k (src/field_kernels.cu:229-256): keep |n_d| <= s_d/(p+1) on every axis
+__device__ __forceinline__ bool keep_mode(const Geom &g, int p, int kx, int ky, int kz)
+{
+    if (p <= 0) return true;
+    const int p1 = p + 1;
+    const int ny = ky > g.sy / 2 ? ky - g.sy : ky;
+    const int nz = kz > g.sz / 2 ? kz - g.sz : kz;
+    return kx <= g.sx / p1 && abs(ny) <= g.sy / p1 && abs(nz) <= g.sz / p1;
+}
+
+}  // namespace cupss

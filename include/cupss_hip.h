@@ -130,6 +130,13 @@ int cupss_system_get_step(cupss_system s, uint64_t *step);
 int cupss_system_set_step(cupss_system s, uint64_t step);
 /* use the caller's hipStream_t (null = the library's own stream) */
 int cupss_system_set_stream(cupss_system s, void *hip_stream);
+/* Step implementation: 0 (default) = the fused two-kernel 2D step when the
+ * system qualifies (one dynamic field; other fields linear in it), else the
+ * general step program; 1 = always the general step program.  Both run on
+ * the GPU and follow the same semantics. */
+int cupss_system_set_path(cupss_system s, int mode);
+/* 1 if cupss_system_advance would take the fused step */
+int cupss_system_fused_eligible(cupss_system s);
 /* evolver::dt change is not part of the reference API; exposed for sweeps */
 int cupss_system_set_dt(cupss_system s, float dt);
 

@@ -371,7 +371,8 @@ void FN(prepare)(FN(sys) *s)
         }
         FN(precalculate_implicit)(s, F);
     }
-    s->step = 0;
+    /* the step counter (and with it the noise stream) is not reset: the
+     * reference's cuRAND generator continues its sequence */
 }
 
 /* ------------------------------------------------------------ the step */

@@ -18,8 +18,10 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-4
 
 
-def gpu(spec):
-    return build_system(C.System, C.pres, spec)
+def gpu(spec, path=0):
+    g, ids = build_system(C.System, C.pres, spec)
+    g.set_path(path)
+    return g, ids
 
 
 def orc(spec, precision="f32"):
@@ -28,9 +30,11 @@ def orc(spec, precision="f32"):
     return build_system(O.System, O.pres, sp)
 
 
-def compare(spec, steps, fields=None, tol=TOL, report=None):
-    print("parity", spec["dims"], steps, flush=True)
-    g, gid = gpu(spec)
+def compare(spec, steps, fields=None, tol=TOL, report=None, path=0, expect_fused=None):
+    print("parity", spec["dims"], steps, "path", "general" if path else "auto", flush=True)
+    g, gid = gpu(spec, path)
+    if expect_fused is not None:
+        assert g.fused_eligible == expect_fused
     o, oid = orc(spec)
     o64, _ = orc(spec, "f64")
     g.step(steps)
@@ -90,14 +94,21 @@ def test_diffusion_config1_closed_form_and_oracle():
     assert err < 1e-4
 
 
+@pytest.mark.parametrize("path", [0, 1])
 @pytest.mark.parametrize("n", [64, 128])
-def test_cahn_hilliard_2d(n):
-    compare(spec_cahn_hilliard((n, n, 1)), 100)
+def test_cahn_hilliard_2d(n, path):
+    compare(spec_cahn_hilliard((n, n, 1)), 100, path=path, expect_fused=True)
 
 
-def test_cahn_hilliard_2d_1024_parity():
+@pytest.mark.parametrize("path", [0, 1])
+def test_cahn_hilliard_2d_1024_parity(path):
     """Config 2 at its full size, 100 steps."""
-    compare(spec_cahn_hilliard((1024, 1024, 1)), 100)
+    compare(spec_cahn_hilliard((1024, 1024, 1)), 100, path=path)
+
+
+@pytest.mark.parametrize("dims", [(256, 32, 1), (32, 512, 1), (2048, 16, 1)])
+def test_cahn_hilliard_2d_rectangular_fused(dims):
+    compare(spec_cahn_hilliard(dims), 60, expect_fused=True)
 
 
 def test_cahn_hilliard_3d():
@@ -110,13 +121,41 @@ def test_cahn_hilliard_1d_and_anisotropic():
     compare(spec_cahn_hilliard((16, 8, 32)), 100)
 
 
+@pytest.mark.parametrize("path", [0, 1])
 @pytest.mark.parametrize("lam", [0.0, 1.5])
-def test_kpz_and_edwards_wilkinson_with_noise(lam):
-    compare(spec_kpz(64, nu=0.8, lam=lam, D=0.01, dt=0.005), 100)
+def test_kpz_and_edwards_wilkinson_with_noise(lam, path):
+    compare(spec_kpz(64, nu=0.8, lam=lam, D=0.01, dt=0.005), 100, path=path, expect_fused=True)
+
+
+@pytest.mark.parametrize("n", [128, 512])
+def test_kpz_fused_sizes(n):
+    compare(spec_kpz(n, nu=1.2, lam=1.0, D=0.01, dt=0.005), 50, expect_fused=True)
+
+
+@pytest.mark.parametrize("path", [0, 1])
+def test_reinitialise_keeps_stale_dealiased_copy(path):
+    """The reference re-runs prepareProblem without clearing real_dealiased
+    (examples/06_kpz/06_kpz.cpp:39-44): the first step after a new initial
+    condition reads the previous dealiased field."""
+    spec = spec_cahn_hilliard((64, 64, 1))
+    x1 = np.random.default_rng(9).uniform(-0.2, 0.2, 64 * 64).astype(np.float32)
+    g, gid = gpu(spec, path)
+    o, oid = orc(spec)
+    g.step(7)
+    o.step(7)
+    g.set_real(gid["phi"], x1)
+    o.set_real(oid["phi"], x1)
+    g.prepare()
+    o.prepare()
+    g.step(13)
+    o.step(13)
+    err = rel_l2(g.get_real(gid["phi"]), o.get_real(oid["phi"]))
+    print("  re-initialised CH: gpu-vs-f32 %.3e" % err)
+    assert err < TOL
 
 
 def test_model_h():
-    compare(spec_model_h(64), 50)
+    compare(spec_model_h(64), 50, expect_fused=False)
 
 
 def test_constraint_division_noise_and_nyquist():
