@@ -52,6 +52,42 @@ def diffusion(S, P, dims, x0, dt=0.01, **kw):
     return s, phi
 
 
+def ensemble_members(count=512, seed=0):
+    """Config 4: 512 independent 2D instances, side uniform in {64,128,256,512},
+    Edwards-Wilkinson for even index (lambda = 0) and KPZ for odd index
+    (lambda ~ U[0, 2]), nu ~ U[0.5, 2], D = 0.01, dt = 0.005, h0 = 0."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(rng.choice([64, 128, 256, 512]))
+        nu = float(rng.uniform(0.5, 2.0))
+        lam = float(rng.uniform(0.0, 2.0)) if i % 2 == 1 else 0.0
+        out.append((n, nu, lam))
+    return out
+
+
+def kpz_system(S, P, n, nu, lam, D=0.01, dt=0.005, seed=0, **kw):
+    """dt h + nu*q^2*h = 0.5*lambda*iqxh^2 + 0.5*lambda*iqyh^2, iqxh = iqx*h,
+    iqyh = iqy*h, addNoise(h, "2*D"); lambda = 0 is Edwards-Wilkinson with
+    the parser's "= 0" constant term."""
+    s = S(n, n, 1, 1.0, 1.0, 1.0, dt, **kw)
+    h = s.add_field(True)
+    ix = s.add_field(False)
+    iy = s.add_field(False)
+    s.add_implicit(h, P(-nu, q2n=1))
+    if lam != 0.0:
+        s.add_term(h, [ix, ix], [P(0.5 * lam)])
+        s.add_term(h, [iy, iy], [P(0.5 * lam)])
+    else:
+        s.add_term(h, [], [P(0.0)])
+    s.add_term(ix, [h], [P(1.0, iqx=1)])
+    s.add_term(iy, [h], [P(1.0, iqy=1)])
+    s.set_noise(h, P(2 * D), seed)
+    s.set_real(h, np.zeros(n * n, np.float32))
+    s.prepare()
+    return s, h
+
+
 WORKLOADS = {
     "ch2d": dict(build=cahn_hilliard, dims=(1024, 1024, 1), name="2D Cahn-Hilliard 1024^2 (single instance)",
                  cpu_steps=30),
@@ -59,7 +95,45 @@ WORKLOADS = {
                  cpu_steps=1),
     "diffusion": dict(build=diffusion, dims=(256, 256, 1), name="2D diffusion 256^2 (reference example 01)",
                       cpu_steps=2000),
+    "ensemble": dict(name="batched EW/KPZ ensemble, 512 instances of 64^2..512^2", cpu_steps=0),
 }
+
+
+def bench_ensemble(args, C):
+    """Batched (one row + one column launch per size class per step, hipGraph
+    replay) against the plain per-solve path (each system advanced on its own,
+    one after the other)."""
+    members = ensemble_members()
+    systems = []
+    for i, (n, nu, lam) in enumerate(members):
+        s, h = kpz_system(C.System, C.pres, n, nu, lam, seed=1000 + i)
+        s.set_noise_mode(0 if args.noise == "real" else 1)
+        systems.append(s)
+    npts = sum(n * n for n, _, _ in members)
+    # plain path: a sample of 8 systems per size, each alone
+    plain_t, plain_pts = 0.0, 0
+    k_plain = max(10, args.steps // 10)
+    for size in (64, 128, 256, 512):
+        idx = [i for i, m in enumerate(members) if m[0] == size][:8]
+        for i in idx:
+            systems[i].step(2)
+            systems[i].synchronize()
+            ms = systems[i].time_steps(k_plain)
+            plain_t += ms / 1e3 * (sum(1 for m in members if m[0] == size) / len(idx))
+            plain_pts += 0
+    plain_value = npts * k_plain / plain_t
+    batch = C.Batch(graph_steps=args.graph)
+    for s in systems:
+        batch.add(s)
+    print("bench: ensemble of %d instances, %.1fM points, noise %s, graph %d" %
+          (len(members), npts / 1e6, args.noise, args.graph), file=sys.stderr, flush=True)
+    batch.step(args.warmup)
+    batch.synchronize()
+    ms = batch.time_steps(args.steps)
+    value = npts * args.steps / (ms / 1e3)
+    return value, ms, npts, {"plain_value": plain_value, "speedup_vs_plain": value / plain_value,
+                             "instances": len(members), "points": npts, "noise": args.noise,
+                             "graph_steps": args.graph}
 
 
 def cpu_baseline(wl, x0):
@@ -96,6 +170,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--path", default="auto", choices=["auto", "general"],
                     help="auto: fused two-kernel step when the system qualifies; general: step program")
+    ap.add_argument("--noise", default="spectral", choices=["real", "spectral"],
+                    help="ensemble noise draw: real-space Philox + R2C, or spectral-space Philox")
+    ap.add_argument("--graph", type=int, default=20, help="ensemble: steps per hipGraph (0 = direct launches)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build runs on a single GPU (--gpus 1); "
@@ -104,6 +181,19 @@ def main():
 
     import cupss_amd as C
     wl = WORKLOADS[args.workload]
+    if args.workload == "ensemble":
+        value, ms, npts, extra = bench_ensemble(args, C)
+        achieved = BYTES_PER_POINT_STEP * value / 1e9
+        out = {"metric": "grid-point-steps/s", "value": value, "unit": "grid-point-steps/s", "n_gpus": 1,
+               "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms / args.steps,
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+               "data": "synthetic", "config": dict({"workload": wl["name"], "dt": 0.005,
+                                                     "path": "batched fused step"}, **extra),
+               "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                            "per": "whole step of the ensemble, 8 B/point algorithmic"}}
+        print(json.dumps(out), flush=True)
+        return
     dims = wl["dims"]
     n = dims[0] * dims[1] * dims[2]
     x0 = np.random.default_rng(1).uniform(-0.1, 0.1, n).astype(np.float32)

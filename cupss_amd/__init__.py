@@ -67,6 +67,15 @@ _SIGS = {
     "cupss_system_set_dt": (_I, [_VP, _F]),
     "cupss_system_set_path": (_I, [_VP, _I]),
     "cupss_system_fused_eligible": (_I, [_VP]),
+    "cupss_system_set_noise_mode": (_I, [_VP, _I]),
+    "cupss_batch_create": (_I, [_VP, ctypes.POINTER(_VP)]),
+    "cupss_batch_destroy": (_I, [_VP]),
+    "cupss_batch_add": (_I, [_VP, _VP]),
+    "cupss_batch_size": (_I, [_VP]),
+    "cupss_batch_set_graph_steps": (_I, [_VP, _I]),
+    "cupss_batch_advance": (_I, [_VP, _I]),
+    "cupss_batch_time_advance": (_I, [_VP, _I, _FP]),
+    "cupss_batch_synchronize": (_I, [_VP]),
     "cupss_field_spectrum_device": (_I, [_VP, _I, ctypes.POINTER(_VP)]),
     "cupss_r2c": (_I, [_I, _I, _I, _VP, _VP, _VP]),
     "cupss_c2r": (_I, [_I, _I, _I, _VP, _I, _F, _VP, _VP, _VP]),
@@ -227,6 +236,10 @@ class System:
         """0 = fused two-kernel step when eligible, 1 = general step program."""
         check(self.L.cupss_system_set_path(self.h, int(mode)))
 
+    def set_noise_mode(self, mode):
+        """0 = real-space Philox + R2C (the reference's draw), 1 = spectral-space draw."""
+        check(self.L.cupss_system_set_noise_mode(self.h, int(mode)))
+
     @property
     def fused_eligible(self):
         return bool(self.L.cupss_system_fused_eligible(self.h))
@@ -239,3 +252,41 @@ class System:
         v = ctypes.c_uint64()
         check(self.L.cupss_system_get_step(self.h, ctypes.byref(v)))
         return v.value
+
+
+class Batch:
+    """Many systems advanced together by the batched fused step."""
+
+    def __init__(self, graph_steps=0):
+        self.L = lib()
+        self.h = ctypes.c_void_p()
+        check(self.L.cupss_batch_create(None, ctypes.byref(self.h)))
+        self.members = []
+        if graph_steps:
+            check(self.L.cupss_batch_set_graph_steps(self.h, graph_steps))
+
+    def add(self, system):
+        check(self.L.cupss_batch_add(self.h, system.h))
+        self.members.append(system)
+
+    def step(self, n=1):
+        check(self.L.cupss_batch_advance(self.h, n))
+
+    def time_steps(self, n):
+        ms = ctypes.c_float()
+        check(self.L.cupss_batch_time_advance(self.h, n, ctypes.byref(ms)))
+        return ms.value
+
+    def synchronize(self):
+        check(self.L.cupss_batch_synchronize(self.h))
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.L.cupss_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -147,17 +147,27 @@ __host__ __device__ constexpr int row_elems(int nc)
 __host__ __device__ constexpr long xcd_blocks(long total) { return (total + 7) / 8 * 8; }
 __device__ __forceinline__ long xcd_item(long b, long nblk) { return (b & 7) * (nblk >> 3) + (b >> 3); }
 
-// exp(DIR*2*pi*i*m/L) from the table (L | TW_N, 0 <= m)
-template <int DIR>
+// exp(DIR*2*pi*i*m/L) from a table of exp(-2*pi*i*j/TWS), j < TWS
+// (L | TWS, 0 <= m).  The table lives in global memory (TWS = TW_N) or in
+// the workgroup's LDS (TWS = the largest length the kernel transforms).
+template <int DIR, int TWS = TW_N>
 __device__ __forceinline__ float2 tw_lookup(const float2 *__restrict__ tw, int m, int L)
 {
-    int j = (m * (TW_N / L)) & (TW_N - 1);
+    int j = (m * (TWS / L)) & (TWS - 1);
     float2 w = tw[j];
     return DIR > 0 ? make_float2(w.x, -w.y) : w;
 }
 
+// The device keeps one table per power-of-two size M at offset M of one
+// array (twiddle_sets()); a kernel copies table M into LDS with this.
+template <int M>
+__device__ __forceinline__ void load_twiddles(float2 *__restrict__ dst, const float2 *__restrict__ sets)
+{
+    for (int i = threadIdx.x; i < M; i += blockDim.x) dst[i] = sets[M + i];
+}
+
 // One Stockham stage of radix R with span NS, then recurse.
-template <int N, int E, int DIR, int NS>
+template <int N, int E, int DIR, int NS, int TWS = TW_N>
 struct Stockham {
     static constexpr int T = N / E;
     static constexpr int REM = N / NS;
@@ -174,7 +184,7 @@ struct Stockham {
                 const int k = j & (NS - 1);
                 if constexpr (NS > 1) {
 #pragma unroll
-                    for (int r = 1; r < R; r++) a[r] = cmul(a[r], tw_lookup<DIR>(tw, r * k, NS * R));
+                    for (int r = 1; r < R; r++) a[r] = cmul(a[r], tw_lookup<DIR, TWS>(tw, r * k, NS * R));
                 }
                 Dft<R, DIR>::run(a);
                 const int base = (j - k) * R + k;
@@ -185,7 +195,7 @@ struct Stockham {
 #pragma unroll
             for (int e = 0; e < E; e++) v[e] = lds[lpad(t + e * T)];
             if constexpr (NS * R < N) __syncthreads();
-            Stockham<N, E, DIR, NS * R>::run(v, lds, t, tw);
+            Stockham<N, E, DIR, NS * R, TWS>::run(v, lds, t, tw);
         }
     }
 };
@@ -193,14 +203,14 @@ struct Stockham {
 // Full transform of one line.  On entry v[e] = x[t + e*T]; on exit
 // v[e] = X[t + e*T] and lds[lpad(k)] = X[k] for all k (after a barrier).
 // All threads of the workgroup must call it (it contains barriers).
-template <int N, int E, int DIR>
+template <int N, int E, int DIR, int TWS = TW_N>
 __device__ __forceinline__ void fft_line(float2 (&v)[E], float2 *lds, int t, const float2 *__restrict__ tw)
 {
     if constexpr (N == 1) {
         lds[0] = v[0];
         __syncthreads();
     } else {
-        Stockham<N, E, DIR, 1>::run(v, lds, t, tw);
+        Stockham<N, E, DIR, 1, TWS>::run(v, lds, t, tw);
     }
 }
 

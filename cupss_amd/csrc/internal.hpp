@@ -73,8 +73,12 @@ struct MaskArgs {
 
 // --- launchers (kernels.hip) ---------------------------------------------
 
-// Twiddle table (device) for the current device, created on first use.
+// Twiddle table (device) for the current device, created on first use:
+// exp(-2 pi i j / TW_N), j < TW_N.
 const float2 *twiddles(hipStream_t s);
+// All power-of-two tables: table M (exp(-2 pi i j / M), j < M) at offset M,
+// M = 1, 2, 4, ..., TW_N.
+const float2 *twiddle_sets(hipStream_t s);
 
 enum LoadKind { LOAD_REAL = 0, LOAD_PRODUCT = 1, LOAD_NOISE = 2 };
 struct R2CSource {
@@ -92,5 +96,8 @@ hipError_t r2c(const Geom &g, const R2CSource &src, float2 *out, hipStream_t s);
 hipError_t c2r(const Geom &g, const float2 *in, int mask_p, float scale, float *out, float2 *scratch,
                hipStream_t s);
 hipError_t update(const UpdateArgs &a, hipStream_t s);
+// eta_hat with the law of R2C(xi), xi ~ N(0,1) per cell, from Philox
+hipError_t spectral_noise_fill(const Geom &g, float2 *eta, uint64_t seed, uint64_t step, uint32_t stream,
+                               hipStream_t s);
 
 }  // namespace cupss

@@ -28,7 +28,8 @@ namespace cupss {
 // ------------------------------------------------------------ twiddles
 
 static std::mutex g_tw_mutex;
-static std::vector<float2 *> g_tw;  // per device
+static std::vector<float2 *> g_tw;    // per device
+static std::vector<float2 *> g_tws;   // per device
 
 const float2 *twiddles(hipStream_t s)
 {
@@ -49,6 +50,28 @@ const float2 *twiddles(hipStream_t s)
     }
     (void)s;
     return g_tw[dev];
+}
+
+const float2 *twiddle_sets(hipStream_t s)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lock(g_tw_mutex);
+    if ((int)g_tws.size() <= dev) g_tws.resize(dev + 1, nullptr);
+    if (!g_tws[dev]) {
+        std::vector<float2> h(2 * TW_N, make_float2(0.f, 0.f));
+        for (int M = 1; M <= TW_N; M *= 2)
+            for (int j = 0; j < M; j++) {
+                double a = -2.0 * M_PI * (double)j / (double)M;
+                h[M + j] = make_float2((float)std::cos(a), (float)std::sin(a));
+            }
+        float2 *d = nullptr;
+        if (hipMalloc(&d, sizeof(float2) * 2 * TW_N) != hipSuccess) return nullptr;
+        if (hipMemcpy(d, h.data(), sizeof(float2) * 2 * TW_N, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+        g_tws[dev] = d;
+    }
+    (void)s;
+    return g_tws[dev];
 }
 
 // ------------------------------------------------------------ sources
@@ -419,6 +442,24 @@ __global__ __launch_bounds__(256) void k_update(UpdateArgs a)
         const float2 eta = a.noisy ? a.noise[b.h] : make_float2(0.f, 0.f);
         a.out[b.h] = finish_update(a, b, a.cur[b.h], acc, eta);
     }
+}
+
+// eta_hat drawn directly in spectral space (see spectral_noise)
+__global__ __launch_bounds__(256) void k_spectral_noise(float2 *__restrict__ eta, Geom g, uint32_t slo, uint32_t shi,
+                                                        uint32_t stream, uint32_t k0, uint32_t k1)
+{
+    const long row = blockIdx.x;
+    const int ky = (int)(row % g.sy), kz = (int)(row / g.sy);
+    for (int kx = threadIdx.x; kx < g.nc1; kx += blockDim.x)
+        eta[row * g.nc1 + kx] = spectral_noise(g, kx, ky, kz, slo, shi, stream, k0, k1);
+}
+
+hipError_t spectral_noise_fill(const Geom &g, float2 *eta, uint64_t seed, uint64_t step, uint32_t stream,
+                               hipStream_t s)
+{
+    hipLaunchKernelGGL(k_spectral_noise, dim3((unsigned)((long)g.sy * g.sz)), dim3(256), 0, s, eta, g,
+                       (uint32_t)step, (uint32_t)(step >> 32), stream, (uint32_t)seed, (uint32_t)(seed >> 32));
+    return hipGetLastError();
 }
 
 hipError_t update(const UpdateArgs &a, hipStream_t s)

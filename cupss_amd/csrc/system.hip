@@ -27,78 +27,17 @@
 #include "fused.hpp"
 #include "internal.hpp"
 #include "philox.hpp"
+#include "system_impl.hpp"
 
 using namespace cupss;
 
 static thread_local std::string g_last_error;
 
-static int fail(int code, const std::string &msg)
+int cupss::fail(int code, const std::string &msg)
 {
     g_last_error = msg;
     return code;
 }
-
-#define HIP_TRY(expr)                                                                          \
-    do {                                                                                       \
-        hipError_t _e = (expr);                                                                \
-        if (_e != hipSuccess)                                                                  \
-            return fail(CUPSS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));      \
-    } while (0)
-
-struct FieldRec {
-    int dynamic = 0;
-    int noisy = 0;
-    cupss_pres noise{};
-    uint64_t seed = 0;
-    std::vector<cupss_pres> imp;
-    std::vector<int> terms;
-    int order = 0;  // dealias order, 0 = none
-    float2 *spec[2] = {nullptr, nullptr};
-    int cur = 0;
-    float *dreal = nullptr;
-    // dealiased real copy: 0 = zero (never updated, the reference's step-0
-    // state), 1 = dreal holds C2R(mask*spec), 2 = same but not yet written
-    // to dreal (after fused steps), 3 = stale (spec reset by set_real after
-    // updates; the reference keeps the old real_dealiased then)
-    int dstate = 0;
-    float2 *eta = nullptr;
-    cupss_real_hook real_hook = nullptr;
-    cupss_spec_hook spec_hook = nullptr;
-    void *hook_user = nullptr;
-};
-
-struct TermRec {
-    int field = 0;
-    std::vector<int> fac;
-    std::vector<cupss_pres> pres;
-    float2 *xhat = nullptr;
-};
-
-struct cupss_system_t {
-    Geom g{};
-    float dx = 1, dy = 1, dz = 1, dt = 0;
-    std::vector<FieldRec> f;
-    std::vector<TermRec> t;
-    float2 *scratch = nullptr;
-    float *rtmp = nullptr;
-    float2 *full = nullptr;   // float2[N] for hooks
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    bool prepared = false;
-    int path_mode = 0;          // 0 = fused step when eligible, 1 = general step only
-    struct Plan {
-        bool ok = false;
-        int phi = -1;
-        std::vector<int> real;                 // fields read in real space by the products
-        std::vector<std::vector<int>> groups;  // phi's product terms with equal prefactors
-        std::vector<int> derived;              // constraint fields psi = M_psi phi
-    } plan;
-    float2 *fz[FUSED_MAX_REAL] = {nullptr, nullptr, nullptr};
-    float2 *fx[FUSED_MAX_GROUPS] = {nullptr, nullptr, nullptr, nullptr};
-    long fused_steps = 0;       // steps taken by the fused path (diagnostics)
-    uint64_t step = 0;
-    int device = 0;
-};
 
 static bool is_pow2(int v) { return v >= 1 && (v & (v - 1)) == 0; }
 
@@ -167,6 +106,7 @@ extern "C" int cupss_system_destroy(cupss_system s)
 {
     if (!s) return CUPSS_OK;
     if (s->stream) hipStreamSynchronize(s->stream);
+    fused_release(s);
     for (auto &F : s->f) {
         hipFree(F.spec[0]);
         hipFree(F.spec[1]);
@@ -199,6 +139,7 @@ extern "C" int cupss_field_create(cupss_system s, int dynamic, int *field_id)
     s->f.push_back(F);
     if (field_id) *field_id = (int)s->f.size() - 1;
     s->prepared = false;
+    s->gen++;
     return CUPSS_OK;
 }
 
@@ -207,6 +148,7 @@ extern "C" int cupss_field_add_implicit(cupss_system s, int field, cupss_pres p)
     if (int r = check_field(s, field)) return r;
     if ((int)s->f[field].imp.size() >= MAX_IMP) return fail(CUPSS_ERR_UNSUPPORTED, "too many implicit terms");
     s->f[field].imp.push_back(p);
+    s->gen++;
     return CUPSS_OK;
 }
 
@@ -215,6 +157,7 @@ extern "C" int cupss_field_set_implicits(cupss_system s, int field, const cupss_
     if (int r = check_field(s, field)) return r;
     if (n < 0 || n > MAX_IMP || (n > 0 && !p)) return fail(CUPSS_ERR_INVALID, "bad implicit list");
     s->f[field].imp.assign(p, p + n);
+    s->gen++;
     return CUPSS_OK;
 }
 
@@ -243,6 +186,7 @@ extern "C" int cupss_term_create(cupss_system s, int field, const int *factors, 
     s->f[field].terms.push_back((int)s->t.size() - 1);
     if (term_id) *term_id = (int)s->t.size() - 1;
     s->prepared = false;
+    s->gen++;
     return CUPSS_OK;
 }
 
@@ -255,6 +199,7 @@ extern "C" int cupss_term_set_prefactors(cupss_system s, int term, const cupss_p
     if (field_pres_count(s, T.field, npres - (int)T.pres.size()) > MAX_PRES)
         return fail(CUPSS_ERR_UNSUPPORTED, "too many prefactors");
     T.pres.assign(pres, pres + npres);
+    s->gen++;
     return CUPSS_OK;
 }
 
@@ -265,6 +210,7 @@ extern "C" int cupss_field_set_noise(cupss_system s, int field, cupss_pres ampli
     s->f[field].noise = amplitude;
     s->f[field].seed = seed;
     s->prepared = false;
+    s->gen++;
     return CUPSS_OK;
 }
 
@@ -275,10 +221,10 @@ extern "C" int cupss_field_set_hooks(cupss_system s, int field, cupss_real_hook 
     s->f[field].real_hook = real_hook;
     s->f[field].spec_hook = spec_hook;
     s->f[field].hook_user = user;
+    s->gen++;
     return CUPSS_OK;
 }
 
-static int materialize(cupss_system s, int fi);
 static void plan_fused(cupss_system s);
 
 extern "C" int cupss_field_set_real(cupss_system s, int field, const float *host)
@@ -390,6 +336,7 @@ extern "C" int cupss_system_prepare(cupss_system s)
                 return fail(CUPSS_ERR_NOMEM, "device allocation failed");
     HIP_TRY(hipStreamSynchronize(s->stream));
     plan_fused(s);
+    s->gen++;
     s->prepared = true;
     return CUPSS_OK;
 }
@@ -403,7 +350,7 @@ extern "C" int cupss_field_aliasing_order(cupss_system s, int field)
 // ------------------------------------------------------------ the step
 
 // host-side precomputation of one field's update descriptor
-static int build_update(cupss_system s, int fi, UpdateArgs &a)
+int cupss::build_update(cupss_system s, int fi, UpdateArgs &a)
 {
     FieldRec &F = s->f[fi];
     std::memset(&a, 0, sizeof(a));
@@ -567,7 +514,7 @@ static int post_update(cupss_system s, int fi)
 }
 
 // write the dealiased copy of a field whose copy is current but not stored
-static int materialize(cupss_system s, int fi)
+int cupss::materialize(cupss_system s, int fi)
 {
     FieldRec &F = s->f[fi];
     if (F.dstate != 2 || !F.dreal || F.order <= 0) return CUPSS_OK;
@@ -584,6 +531,10 @@ static int run_phase(cupss_system s, int dynamic)
     for (int i = 0; i < nf; i++) {
         FieldRec &F = s->f[i];
         if (F.dynamic != dynamic || !F.noisy) continue;
+        if (s->noise_mode == 1) {
+            HIP_TRY(spectral_noise_fill(g, F.eta, F.seed, s->step, (uint32_t)i, s->stream));
+            continue;
+        }
         R2CSource src{};
         src.kind = LOAD_NOISE;
         src.seed = F.seed;
@@ -622,7 +573,7 @@ static int run_phase(cupss_system s, int dynamic)
     return CUPSS_OK;
 }
 
-static int general_steps(cupss_system s, int nsteps)
+int cupss::general_steps(cupss_system s, int nsteps)
 {
     for (int i = 0; i < (int)s->f.size(); i++)
         if (int r = materialize(s, i)) return r;
@@ -641,7 +592,6 @@ static bool hooks_present(cupss_system s)
     return false;
 }
 
-static int fused_steps(cupss_system s, int nsteps);
 
 extern "C" int cupss_system_advance(cupss_system s, int nsteps)
 {
@@ -667,6 +617,15 @@ extern "C" int cupss_system_set_path(cupss_system s, int mode)
     return CUPSS_OK;
 }
 
+extern "C" int cupss_system_set_noise_mode(cupss_system s, int mode)
+{
+    if (int r = check_sys(s)) return r;
+    if (mode != 0 && mode != 1) return fail(CUPSS_ERR_INVALID, "noise mode is 0 (real space) or 1 (spectral)");
+    s->noise_mode = mode;
+    s->gen++;
+    return CUPSS_OK;
+}
+
 extern "C" int cupss_system_fused_eligible(cupss_system s)
 {
     if (check_sys(s)) return 0;
@@ -689,7 +648,7 @@ static bool same_pres(const std::vector<cupss_pres> &a, const std::vector<cupss_
 static void plan_fused(cupss_system s)
 {
     auto &P = s->plan;
-    P = cupss_system_t::Plan();
+    P = FusedPlan();
     if (!fused_supported(s->g.sx, s->g.sy, s->g.sz)) return;
     int phi = -1;
     for (int i = 0; i < (int)s->f.size(); i++)
@@ -745,120 +704,7 @@ static void plan_fused(cupss_system s)
     P.ok = true;
 }
 
-static int fused_steps(cupss_system s, int nsteps)
-{
-    const auto &P = s->plan;
-    const Geom &g = s->g;
-    FieldRec &F = s->f[P.phi];
-    for (size_t i = 0; i < P.real.size(); i++)
-        if (!s->fz[i] && hipMalloc(&s->fz[i], sizeof(float2) * g.nh) != hipSuccess)
-            return fail(CUPSS_ERR_NOMEM, "device allocation failed");
-    for (size_t i = 0; i < P.groups.size(); i++)
-        if (!s->fx[i] && hipMalloc(&s->fx[i], sizeof(float2) * g.nh) != hipSuccess)
-            return fail(CUPSS_ERR_NOMEM, "device allocation failed");
 
-    // column-pass arguments: phi's update with grouped product terms
-    FusedColArgs ca;
-    std::memset(&ca, 0, sizeof(ca));
-    build_update(s, P.phi, ca.u);
-    ca.u.noise = F.eta;
-    ca.ngroups = (int)P.groups.size();
-    for (size_t gi = 0; gi < P.groups.size(); gi++) {
-        ca.X[gi] = s->fx[gi];
-        for (size_t k = 0; k < F.terms.size(); k++)
-            for (int ti : P.groups[gi])
-                if (F.terms[k] == ti) {
-                    ca.u.terms[k].kind = TERM_GROUPED;
-                    if (ti == P.groups[gi][0]) ca.gterm[gi] = (int)k;
-                }
-    }
-    ca.nreal = (int)P.real.size();
-    int lp = 0;
-    for (size_t i = 0; i < P.real.size(); i++) {
-        const int f = P.real[i];
-        ca.p[i] = s->f[f].order;
-        ca.Z[i] = s->fz[i];
-        ca.zero[i] = (f == P.phi && F.dstate == 0) ? 1 : 0;
-        if (f != P.phi) {
-            const TermRec &T = s->t[s->f[f].terms[0]];
-            const int m = (T.pres[0].iqx + T.pres[0].iqy + T.pres[0].iqz) % 2;
-            ca.lmul[i].mult_i = m;
-            ca.lmul[i].pres0 = lp;
-            ca.lmul[i].npres = (int)T.pres.size();
-            for (const cupss_pres &p : T.pres) {
-                const int iu = p.iqx + p.iqy + p.iqz;
-                PresDev &D = ca.lpres[lp++];
-                D.c = p.preFactor * (float)(-2 * (((iu - m) / 2) % 2) + 1);
-                D.q2n = p.q2n;
-                D.a = p.iqx;
-                D.b = p.iqy;
-                D.cz = p.iqz;
-                D.invq = p.invq;
-            }
-        }
-    }
-    // row-pass arguments
-    FusedRowArgs ra;
-    std::memset(&ra, 0, sizeof(ra));
-    ra.nrows = (long)g.sy * g.sz;
-    ra.scale = 1.0f / (float)g.n;
-    ra.nreal = ca.nreal;
-    for (int i = 0; i < ca.nreal; i++) ra.Z[i] = s->fz[i];
-    ra.ngroups = ca.ngroups;
-    int np = 0;
-    for (size_t gi = 0; gi < P.groups.size(); gi++) {
-        ra.g0[gi] = np;
-        ra.X[gi] = s->fx[gi];
-        for (int ti : P.groups[gi]) {
-            const TermRec &T = s->t[ti];
-            ra.nfac[np] = (int)T.fac.size();
-            for (size_t c = 0; c < T.fac.size(); c++)
-                ra.fac[np][c] = (int)(std::find(P.real.begin(), P.real.end(), T.fac[c]) - P.real.begin());
-            np++;
-        }
-    }
-    ra.g0[P.groups.size()] = np;
-    ra.noisy = F.noisy;
-    ra.eta = F.eta;
-    ra.key0 = (uint32_t)F.seed;
-    ra.key1 = (uint32_t)(F.seed >> 32);
-    ra.stream = (uint32_t)P.phi;
-
-    // prologue: the real fields of the first step from the current state
-    ca.spec_in = F.spec[F.cur];
-    ca.spec_out = F.spec[F.cur ^ 1];
-    ca.do_update = 0;
-    if (ca.nreal > 0) HIP_TRY(fused_cols(ca, g.sy, s->stream));
-    for (int i = 0; i < ca.nreal; i++) ca.zero[i] = 0;
-    ca.do_update = 1;
-    const bool need_rows = ca.nreal > 0 || ra.noisy;
-    for (int n = 0; n < nsteps; n++) {
-        if (need_rows) {
-            ra.step_lo = (uint32_t)s->step;
-            ra.step_hi = (uint32_t)(s->step >> 32);
-            HIP_TRY(fused_rows(ra, g.nc, s->stream));
-        }
-        ca.spec_in = F.spec[F.cur];
-        ca.spec_out = F.spec[F.cur ^ 1];
-        HIP_TRY(fused_cols(ca, g.sy, s->stream));
-        F.cur ^= 1;
-        s->step += 1;
-    }
-    s->fused_steps += nsteps;
-    // state of the general path: phi's dealiased copy is current; each
-    // derived field holds P(M_psi phi) of the last step's starting phi
-    if (F.order > 0) F.dstate = 2;
-    for (int d : P.derived) {
-        FieldRec &D = s->f[d];
-        UpdateArgs a;
-        build_update(s, d, a);
-        a.terms[0].X = F.spec[F.cur ^ 1];
-        a.out = D.spec[D.cur];
-        HIP_TRY(update(a, s->stream));
-        if (D.order > 0) D.dstate = 2;
-    }
-    return CUPSS_OK;
-}
 
 extern "C" int cupss_system_time_advance(cupss_system s, int nsteps, float *ms)
 {
@@ -918,6 +764,7 @@ extern "C" int cupss_system_set_dt(cupss_system s, float dt)
 {
     if (int r = check_sys(s)) return r;
     s->dt = dt;
+    s->gen++;
     return CUPSS_OK;
 }
 

@@ -1,0 +1,92 @@
+// system_impl.hpp -- internal state of a cupss_system (the device side of
+// one evolver) shared by system.hip and batch.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "fused.hpp"
+#include "internal.hpp"
+
+namespace cupss {
+
+int fail(int code, const std::string &msg);
+
+#define HIP_TRY(expr)                                                                                  \
+    do {                                                                                               \
+        hipError_t _e = (expr);                                                                        \
+        if (_e != hipSuccess)                                                                          \
+            return ::cupss::fail(CUPSS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));     \
+    } while (0)
+
+struct FieldRec {
+    int dynamic = 0;
+    int noisy = 0;
+    cupss_pres noise{};
+    uint64_t seed = 0;
+    std::vector<cupss_pres> imp;
+    std::vector<int> terms;
+    int order = 0;  // dealias order, 0 = none
+    float2 *spec[2] = {nullptr, nullptr};
+    int cur = 0;
+    float *dreal = nullptr;
+    // dealiased real copy: 0 = zero (never updated, the reference's step-0
+    // state), 1 = dreal holds C2R(mask*spec), 2 = same but not yet written
+    // to dreal (after fused steps), 3 = stale (spec reset by set_real after
+    // updates; the reference keeps the old real_dealiased then)
+    int dstate = 0;
+    float2 *eta = nullptr;
+    cupss_real_hook real_hook = nullptr;
+    cupss_spec_hook spec_hook = nullptr;
+    void *hook_user = nullptr;
+};
+
+struct TermRec {
+    int field = 0;
+    std::vector<int> fac;
+    std::vector<cupss_pres> pres;
+    float2 *xhat = nullptr;
+};
+
+struct FusedPlan {
+    bool ok = false;
+    int phi = -1;
+    std::vector<int> real;                 // fields read in real space by the products
+    std::vector<std::vector<int>> groups;  // phi's product terms with equal prefactors
+    std::vector<int> derived;              // constraint fields psi = M_psi phi
+};
+
+}  // namespace cupss
+
+struct cupss_system_t {
+    cupss::Geom g{};
+    float dx = 1, dy = 1, dz = 1, dt = 0;
+    std::vector<cupss::FieldRec> f;
+    std::vector<cupss::TermRec> t;
+    float2 *scratch = nullptr;
+    float *rtmp = nullptr;
+    float2 *full = nullptr;   // float2[N] for hooks
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    bool prepared = false;
+    int path_mode = 0;        // 0 = fused step when eligible, 1 = general step only
+    int noise_mode = 0;       // 0 = real-space Philox + R2C, 1 = spectral-space Philox
+    cupss::FusedPlan plan;
+    float2 *fz[cupss::FUSED_MAX_REAL] = {nullptr, nullptr, nullptr};
+    float2 *fx[cupss::FUSED_MAX_GROUPS] = {nullptr, nullptr, nullptr, nullptr};
+    long fused_steps = 0;     // steps taken by the fused path (diagnostics)
+    uint64_t step = 0;
+    int device = 0;
+    void *batch = nullptr;    // internal one-instance fused batch
+    uint64_t gen = 1;         // bumped by every change that alters the step program
+};
+
+namespace cupss {
+int build_update(cupss_system s, int fi, UpdateArgs &a);
+int materialize(cupss_system s, int fi);
+int general_steps(cupss_system s, int nsteps);
+// fused path (batch.hip)
+int fused_steps(cupss_system s, int nsteps);
+void fused_release(cupss_system s);
+}  // namespace cupss

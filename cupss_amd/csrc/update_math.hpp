@@ -12,6 +12,7 @@
 
 #include "fft_core.hpp"
 #include "internal.hpp"
+#include "philox.hpp"
 
 namespace cupss {
 
@@ -166,6 +167,36 @@ __device__ __forceinline__ bool keep_mode(const Geom &g, int p, int kx, int ky, 
     const int ny = ky > g.sy / 2 ? ky - g.sy : ky;
     const int nz = kz > g.sz / 2 ? kz - g.sz : kz;
     return kx <= g.sx / p1 && abs(ny) <= g.sy / p1 && abs(nz) <= g.sz / p1;
+}
+
+// The law of R2C(xi) for i.i.d. xi ~ N(0,1) on N cells: at each bin a
+// complex Gaussian with E|eta|^2 = N (re and im independent, variance N/2),
+// independent between conjugate pairs, real with variance N at
+// self-conjugate bins.  Bins with kx = 0 or nc have their partner
+// (kx, -ky, -kz) inside the half spectrum: the partner with the smaller
+// (kz, ky) draws, the other takes the conjugate.
+__device__ __forceinline__ float2 spectral_noise(const Geom &g, int kx, int ky, int kz, uint32_t slo, uint32_t shi,
+                                                 uint32_t stream, uint32_t k0, uint32_t k1)
+{
+    bool conj = false, real = false;
+    int cy = ky, cz = kz;
+    if (kx == 0 || kx == g.nc) {
+        const int py = (g.sy - ky) % g.sy, pz = (g.sz - kz) % g.sz;
+        if (py == ky && pz == kz) {
+            real = true;
+        } else if (pz < kz || (pz == kz && py < ky)) {
+            cy = py;
+            cz = pz;
+            conj = true;
+        }
+    }
+    const uint32_t hc = (uint32_t)(((long)cz * g.sy + cy) * g.nc1 + kx);
+    const Philox4 o = philox4x32_10(hc, slo, shi, stream | 0x80000000u, k0, k1);
+    const float2 z = box_muller(o.x[0], o.x[1]);
+    const float n = (float)g.n;
+    if (real) return make_float2(sqrtf(n) * z.x, 0.0f);
+    const float s = sqrtf(0.5f * n);
+    return make_float2(s * z.x, conj ? -s * z.y : s * z.y);
 }
 
 }  // namespace cupss

@@ -135,10 +135,37 @@ int cupss_system_set_stream(cupss_system s, void *hip_stream);
  * general step program; 1 = always the general step program.  Both run on
  * the GPU and follow the same semantics. */
 int cupss_system_set_path(cupss_system s, int mode);
+/* Noise draw: 0 (default) = xi ~ N(0,1) per real-space cell, then R2C (the
+ * reference's createNoise, src/field.cpp:300-316; the oracle reproduces it
+ * pointwise); 1 = eta_hat drawn directly in spectral space with the same
+ * law (complex Gaussian, E|eta|^2 = N, Hermitian), which saves the
+ * transform.  Not in the reference. */
+int cupss_system_set_noise_mode(cupss_system s, int mode);
 /* 1 if cupss_system_advance would take the fused step */
 int cupss_system_fused_eligible(cupss_system s);
 /* evolver::dt change is not part of the reference API; exposed for sweeps */
 int cupss_system_set_dt(cupss_system s, float dt);
+
+/* ---- batched path ---------------------------------------------------
+ * Many independent systems advanced together: per step, one row and one
+ * column launch per size class of systems (and one elementwise launch for
+ * systems that need no transform), optionally replayed from a hipGraph.
+ * Member systems must be prepared, qualify for the fused step (one dynamic
+ * field, other fields linear in it, no callbacks) and must not be advanced
+ * on their own while in the batch.  Each keeps its own parameters, step
+ * counter and noise stream, so a system advanced in a batch follows the
+ * same sequence as when advanced alone.  No counterpart in the reference
+ * (which steps one evolver at a time, src/evolver.cpp:199-226). */
+typedef struct cupss_batch_t *cupss_batch;
+int cupss_batch_create(void *hip_stream, cupss_batch *out);
+int cupss_batch_destroy(cupss_batch b);
+int cupss_batch_add(cupss_batch b, cupss_system s);
+int cupss_batch_size(cupss_batch b);
+/* 0 = direct launches; K (even, >= 2) = capture K steps in a hipGraph */
+int cupss_batch_set_graph_steps(cupss_batch b, int k);
+int cupss_batch_advance(cupss_batch b, int nsteps);
+int cupss_batch_time_advance(cupss_batch b, int nsteps, float *ms);
+int cupss_batch_synchronize(cupss_batch b);
 
 /* Device buffer access for callers that bring their own kernels (e.g. the
  * reference's user callbacks that launch __global__ functions): device

@@ -1,0 +1,110 @@
+"""GPU: the batched fused path.
+
+* Every system advanced inside a batch of mixed sizes, models and
+  parameters ends bit-identical to the same system advanced alone (same
+  kernels, same per-system parameters and noise streams).
+* hipGraph replay equals direct launches bit for bit.
+* Edwards-Wilkinson: the stationary structure factor of the scheme,
+  S(q) = <|h_q|^2>/N = D / (dV nu q^2 (1 + dt nu q^2 / 2)), is met by both
+  noise draws (real-space Philox + R2C, and the spectral-space draw).
+* KPZ: the two noise draws give the same structure factor.
+"""
+import numpy as np
+import pytest
+
+import cupss_amd as C
+from tests.helpers import build_system, qgrid, spec_cahn_hilliard, spec_kpz
+
+pytestmark = pytest.mark.gpu
+
+
+def make(spec, noise_mode=0):
+    g, ids = build_system(C.System, C.pres, spec)
+    g.set_noise_mode(noise_mode)
+    return g, ids
+
+
+def mixed_specs():
+    rng = np.random.default_rng(4)
+    specs = []
+    for i, n in enumerate([64, 128, 64, 256, 128, 32, 512, 64]):
+        if i % 3 == 0:
+            specs.append(spec_cahn_hilliard((n, n, 1), seed=i))
+        else:
+            lam = 0.0 if i % 3 == 1 else float(rng.uniform(0, 2))
+            specs.append(spec_kpz(n, nu=float(rng.uniform(0.5, 2)), lam=lam, D=0.01, dt=0.005, seed=100 + i))
+    return specs
+
+
+@pytest.mark.parametrize("noise_mode", [0, 1])
+@pytest.mark.parametrize("graph", [0, 4])
+def test_batch_equals_alone(noise_mode, graph):
+    specs = mixed_specs()
+    print("batch", len(specs), "noise mode", noise_mode, "graph", graph, flush=True)
+    alone = []
+    for sp in specs:
+        g, ids = make(sp, noise_mode)
+        g.step(23)
+        alone.append(g.get_real(ids["phi" if "phi" in ids else "h"]))
+    batch = C.Batch(graph_steps=graph)
+    members = []
+    for sp in specs:
+        g, ids = make(sp, noise_mode)
+        assert g.fused_eligible
+        batch.add(g)
+        members.append((g, ids))
+    batch.step(10)
+    batch.step(13)
+    for (g, ids), ref in zip(members, alone):
+        got = g.get_real(ids["phi" if "phi" in ids else "h"])
+        assert np.array_equal(got, ref)
+        assert g.step_count == 23
+
+
+def ew_structure_factor(n, nu, D, dt, noise_mode, steps, burn, every=1):
+    sp = spec_kpz(n, nu=nu, lam=0.0, D=D, dt=dt, seed=11)
+    g, ids = make(sp, noise_mode)
+    g.step(burn)
+    acc = np.zeros((n, n // 2 + 1))
+    m = 0
+    for _ in range(steps // every):
+        g.step(every)
+        h = g.get_real(ids["h"]).astype(np.float64).reshape(n, n)
+        acc += np.abs(np.fft.rfft2(h)) ** 2
+        m += 1
+    return acc / m / (n * n)
+
+
+@pytest.mark.parametrize("noise_mode", [0, 1])
+def test_edwards_wilkinson_structure_factor(noise_mode):
+    n, nu, D, dt = 64, 1.0, 0.01, 0.05
+    S = ew_structure_factor(n, nu, D, dt, noise_mode, steps=4000, burn=1500)
+    qx, qy, _ = qgrid(n, n)
+    q2 = (qx ** 2 + qy ** 2)[0][:, : n // 2 + 1]
+    dtf = float(np.float32(dt))
+    theory = np.where(q2 > 0, D / (nu * np.maximum(q2, 1e-30) * (1 + dtf * nu * q2 / 2)), 0.0)
+    fast = q2 > 0.3       # modes that relax within the burn-in
+    ratio = S[fast] / theory[fast]
+    print("  EW S(q)/theory over %d modes: mean %.4f" % (fast.sum(), ratio.mean()))
+    assert abs(ratio.mean() - 1.0) < 0.01
+
+
+def test_kpz_structure_factor_same_for_both_noise_draws():
+    n = 64
+    res = []
+    for mode in (0, 1):
+        sp = spec_kpz(n, nu=1.0, lam=1.0, D=0.05, dt=0.02, seed=21)
+        g, ids = make(sp, mode)
+        g.step(1000)
+        acc = np.zeros((n, n // 2 + 1))
+        for _ in range(1500):
+            g.step(2)
+            h = g.get_real(ids["h"]).astype(np.float64).reshape(n, n)
+            acc += np.abs(np.fft.rfft2(h)) ** 2
+        res.append(acc / 1500 / (n * n))
+    qx, qy, _ = qgrid(n, n)
+    q2 = (qx ** 2 + qy ** 2)[0][:, : n // 2 + 1]
+    fast = q2 > 0.3
+    r = res[1][fast].sum() / res[0][fast].sum()
+    print("  KPZ S(q) spectral / real-space draw: %.4f" % r)
+    assert abs(r - 1.0) < 0.01
