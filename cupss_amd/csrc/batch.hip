@@ -55,10 +55,14 @@ struct FusedBatch {
     std::vector<int> elem;      // elementwise instances
     FusedBlock *eb = nullptr;
     int neb = 0;
+    FusedBlock *db = nullptr;   // rows of instances with derived fields
+    int ndb = 0;
+    hipEvent_t done = nullptr;  // recorded after each run; member streams wait on it
 
     std::vector<FusedInst> h;
     FusedInst *d = nullptr;
     size_t dcap = 0;
+    FusedInst *pinned = nullptr;  // staging of the descriptor upload
     bool built = false;
 
     int graph_steps = 0;
@@ -80,6 +84,9 @@ struct FusedBatch {
         hipFree(eb);
         eb = nullptr;
         neb = 0;
+        hipFree(db);
+        db = nullptr;
+        ndb = 0;
         elem.clear();
         if (gexec) hipGraphExecDestroy(gexec);
         if (graph) hipGraphDestroy(graph);
@@ -94,7 +101,11 @@ struct FusedBatch {
         release_tables();
         hipFree(d);
         d = nullptr;
+        hipHostFree(pinned);
+        pinned = nullptr;
         dcap = 0;
+        if (done) hipEventDestroy(done);
+        done = nullptr;
         if (own_stream && stream) hipStreamDestroy(stream);
         stream = nullptr;
     }
@@ -169,6 +180,29 @@ static int describe(cupss_system s, FusedInst &a)
     }
     a.g0[P.groups.size()] = np;
     a.noise_mode = !F.noisy ? NOISE_NONE : (s->noise_mode == 1 ? NOISE_SPECTRAL : NOISE_REAL);
+    if ((int)P.derived.size() > FUSED_MAX_DERIVED) return fail(CUPSS_ERR_UNSUPPORTED, "too many derived fields");
+    int dp = 0;
+    a.nder = (int)P.derived.size();
+    for (size_t d = 0; d < P.derived.size(); d++) {
+        const FieldRec &D = s->f[P.derived[d]];
+        const TermRec &T = s->t[D.terms[0]];
+        const int m = (T.pres[0].iqx + T.pres[0].iqy + T.pres[0].iqz) % 2;
+        a.dout[d] = D.spec[D.cur];
+        a.dmul[d].mult_i = m;
+        a.dmul[d].pres0 = dp;
+        a.dmul[d].npres = (int)T.pres.size();
+        for (const cupss_pres &p : T.pres) {
+            if (dp >= FUSED_MAX_LPRES) return fail(CUPSS_ERR_UNSUPPORTED, "too many derived prefactors");
+            const int iu = p.iqx + p.iqy + p.iqz;
+            PresDev &Q = a.dpres[dp++];
+            Q.c = p.preFactor * (float)(-2 * (((iu - m) / 2) % 2) + 1);
+            Q.q2n = p.q2n;
+            Q.a = p.iqx;
+            Q.b = p.iqy;
+            Q.cz = p.iqz;
+            Q.invq = p.invq;
+        }
+    }
     a.key0 = (uint32_t)F.seed;
     a.key1 = (uint32_t)(F.seed >> 32);
     a.stream = (uint32_t)P.phi;
@@ -201,9 +235,14 @@ static int build(FusedBatch &B)
     for (size_t i = 0; i < B.sys.size(); i++)
         if (int r = describe(B.sys[i], B.h[i])) return r;
     if (B.dcap < B.h.size()) {
+        HIP_TRY(hipStreamSynchronize(B.stream));
         hipFree(B.d);
+        hipHostFree(B.pinned);
         B.d = nullptr;
-        if (hipMalloc(&B.d, sizeof(FusedInst) * std::max<size_t>(B.h.size(), 1)) != hipSuccess)
+        B.pinned = nullptr;
+        const size_t n = std::max<size_t>(B.h.size(), 1);
+        if (hipMalloc(&B.d, sizeof(FusedInst) * n) != hipSuccess ||
+            hipHostMalloc(&B.pinned, sizeof(FusedInst) * n, hipHostMallocDefault) != hipSuccess)
             return fail(CUPSS_ERR_NOMEM, "device allocation failed");
         B.dcap = B.h.size();
     }
@@ -238,12 +277,16 @@ static int build(FusedBatch &B)
         if (int r = upload_blocks(rows, &c.rb, &c.nrb)) return r;
         if (int r = upload_blocks(cols, &c.cb, &c.ncb)) return r;
     }
-    std::vector<FusedBlock> er;
+    std::vector<FusedBlock> er, dr;
     for (int i : B.elem) {
         const Geom &g = B.h[i].g;
         for (int r0 = 0; r0 < g.sy * g.sz; r0++) er.push_back(FusedBlock{i, r0});
     }
+    for (size_t i = 0; i < B.h.size(); i++)
+        if (B.h[i].nder > 0)
+            for (int r0 = 0; r0 < B.h[i].g.sy * B.h[i].g.sz; r0++) dr.push_back(FusedBlock{(int)i, r0});
     if (int r = upload_blocks(er, &B.eb, &B.neb)) return r;
+    if (int r = upload_blocks(dr, &B.db, &B.ndb)) return r;
     B.gens.clear();
     for (cupss_system s : B.sys) B.gens.push_back(s->gen);
     B.built = true;
@@ -259,9 +302,11 @@ static bool stale_tables(const FusedBatch &B)
 }
 
 // refresh the per-run parts of the descriptors (buffers, step counters,
-// step-0 flags) and upload them
+// step-0 flags) and upload them; the previous run must be finished with the
+// device table and the staging buffer before either is rewritten
 static int refresh(FusedBatch &B)
 {
+    HIP_TRY(hipStreamSynchronize(B.stream));
     for (size_t i = 0; i < B.sys.size(); i++) {
         cupss_system s = B.sys[i];
         FieldRec &F = s->f[s->plan.phi];
@@ -270,8 +315,13 @@ static int refresh(FusedBatch &B)
         a.spec[1] = F.spec[F.cur ^ 1];
         a.step_base = s->step;
         for (int k = 0; k < a.nreal; k++) a.zero[k] = (s->plan.real[k] == s->plan.phi && F.dstate == 0) ? 1 : 0;
+        for (int d = 0; d < a.nder; d++) {
+            FieldRec &D = s->f[s->plan.derived[d]];
+            a.dout[d] = D.spec[D.cur];
+        }
     }
-    HIP_TRY(hipMemcpyAsync(B.d, B.h.data(), sizeof(FusedInst) * B.h.size(), hipMemcpyHostToDevice, B.stream));
+    std::memcpy(B.pinned, B.h.data(), sizeof(FusedInst) * B.h.size());
+    HIP_TRY(hipMemcpyAsync(B.d, B.pinned, sizeof(FusedInst) * B.h.size(), hipMemcpyHostToDevice, B.stream));
     return CUPSS_OK;
 }
 
@@ -292,9 +342,15 @@ static int launch_step(FusedBatch &B, int parity, uint32_t rel)
     return CUPSS_OK;
 }
 
-// after a run: the general path's view of every system
+// after a run: the general path's view of every system.  Each derived field
+// holds P(M_psi phi) of the last step's starting phi, i.e. of the spectrum
+// buffer the last step read (parity (nsteps-1) & 1); all of them are
+// written by one launch.  Member streams then wait for the batch stream.
 static int settle(FusedBatch &B, int nsteps)
 {
+    if (B.ndb > 0) HIP_TRY(fused_derived_launch(B.d, B.db, B.ndb, (nsteps - 1) & 1, B.stream));
+    if (!B.done) HIP_TRY(hipEventCreateWithFlags(&B.done, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(B.done, B.stream));
     for (cupss_system s : B.sys) {
         const FusedPlan &P = s->plan;
         FieldRec &F = s->f[P.phi];
@@ -302,16 +358,9 @@ static int settle(FusedBatch &B, int nsteps)
         s->step += (uint64_t)nsteps;
         s->fused_steps += nsteps;
         if (F.order > 0) F.dstate = 2;
-        // each derived field holds P(M_psi phi) of the last step's starting phi
-        for (int dfi : P.derived) {
-            FieldRec &D = s->f[dfi];
-            UpdateArgs a;
-            build_update(s, dfi, a);
-            a.terms[0].X = F.spec[F.cur ^ 1];
-            a.out = D.spec[D.cur];
-            HIP_TRY(update(a, B.stream));
-            if (D.order > 0) D.dstate = 2;
-        }
+        for (int dfi : P.derived)
+            if (s->f[dfi].order > 0) s->f[dfi].dstate = 2;
+        if (s->stream != B.stream) HIP_TRY(hipStreamWaitEvent(s->stream, B.done, 0));
     }
     return CUPSS_OK;
 }

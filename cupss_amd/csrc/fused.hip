@@ -329,6 +329,35 @@ __global__ __launch_bounds__(256) void k_felem(const FusedInst *__restrict__ ins
     }
 }
 
+// Derived constraint fields after a run: psi = P(M_psi phi) from phi's
+// spectrum at the start of the last step (setNotDynamic_k with one
+// single-field term, no implicit operator, no noise).
+__global__ __launch_bounds__(256) void k_fderived(const FusedInst *__restrict__ insts,
+                                                  const FusedBlock *__restrict__ blocks, int parity)
+{
+    const FusedBlock blk = blocks[blockIdx.x];
+    if (blk.inst < 0) return;
+    const FusedInst &a = insts[blk.inst];
+    const Geom &g = a.g;
+    const int ky = blk.line % g.sy, kz = blk.line / g.sy;
+    const float2 *__restrict__ src = a.spec[parity];
+    for (int kx = threadIdx.x; kx < g.nc1; kx += blockDim.x) {
+        const Bin b = make_bin(g, kx, ky, kz);
+        const float2 v = src[b.h];
+        for (int d = 0; d < a.nder; d++) {
+            const TermRefDev &M = a.dmul[d];
+            a.dout[d][b.h] = apply_mult(multiplier(a.dpres, M.pres0, M.npres, M.mult_i, b), M.mult_i, v);
+        }
+    }
+}
+
+hipError_t fused_derived_launch(const FusedInst *inst, const FusedBlock *blocks, int nblocks, int parity,
+                                hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fderived, dim3(nblocks), dim3(256), 0, s, inst, blocks, parity);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------ dispatch
 
 #define FUSED_ROW_CASES(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048)

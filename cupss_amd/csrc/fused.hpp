@@ -14,6 +14,7 @@ constexpr int FUSED_MAX_GROUPS = 4;  // groups of product terms with equal prefa
 constexpr int FUSED_MAX_PRODS = 8;   // product terms over all groups
 constexpr int FUSED_MAX_FAC = 4;     // factors of one product
 constexpr int FUSED_MAX_LPRES = 16;  // prefactors of the derived-field multipliers
+constexpr int FUSED_MAX_DERIVED = 4; // derived fields written back after a run
 
 enum NoiseMode : int {
     NOISE_NONE = 0,
@@ -45,6 +46,11 @@ struct FusedInst {
     int zero[FUSED_MAX_REAL];                 // real field is zero in the prologue (step-0 state)
     TermRefDev lmul[FUSED_MAX_REAL];          // L_f (npres = 0: identity)
     PresDev lpres[FUSED_MAX_LPRES];
+    // derived fields psi_d = P(M_d phi_old), written after a run
+    int nder;
+    float2 *dout[FUSED_MAX_DERIVED];
+    TermRefDev dmul[FUSED_MAX_DERIVED];
+    PresDev dpres[FUSED_MAX_LPRES];
 };
 
 // Work item of one workgroup: instance and first row / column (inst < 0: idle).
@@ -63,6 +69,10 @@ hipError_t fused_rows_launch(int nc, const FusedInst *inst, const FusedBlock *bl
 // mode 0: prologue (real fields of the current state), 1: step
 hipError_t fused_cols_launch(int ny, const FusedInst *inst, const FusedBlock *blocks, int nblocks, int parity,
                              uint32_t step_rel, int mode, hipStream_t s);
+// derived fields of every instance from phi's spectrum buffer [parity]
+// (blocks: one spectral row each)
+hipError_t fused_derived_launch(const FusedInst *inst, const FusedBlock *blocks, int nblocks, int parity,
+                                hipStream_t s);
 // elementwise step of instances with no products and no real-space noise
 // (blocks: one spectral row each)
 hipError_t fused_elem_launch(const FusedInst *inst, const FusedBlock *blocks, int nblocks, int parity,

@@ -55,10 +55,27 @@ def test_batch_equals_alone(noise_mode, graph):
         members.append((g, ids))
     batch.step(10)
     batch.step(13)
-    for (g, ids), ref in zip(members, alone):
+    bad = []
+    for i, ((g, ids), ref) in enumerate(zip(members, alone)):
         got = g.get_real(ids["phi" if "phi" in ids else "h"])
-        assert np.array_equal(got, ref)
+        d = float(np.max(np.abs(got - ref)))
+        print("  member %d %s: max|batch - alone| = %.3e" % (i, g.dims, d))
+        if not np.array_equal(got, ref):
+            bad.append(i)
         assert g.step_count == 23
+    assert not bad
+
+
+def test_split_runs_equal_one_run():
+    """Advancing 10 + 13 steps equals advancing 23 (prologue equivalence)."""
+    for sp in (spec_cahn_hilliard((64, 64, 1), seed=2), spec_kpz(64, nu=1.1, lam=1.3, D=0.01, dt=0.005, seed=5)):
+        key = "phi" if "phi" in dict(sp["fields"]) else "h"
+        a, ia = make(sp)
+        a.step(23)
+        b, ib = make(sp)
+        b.step(10)
+        b.step(13)
+        assert np.array_equal(a.get_real(ia[key]), b.get_real(ib[key]))
 
 
 def ew_structure_factor(n, nu, D, dt, noise_mode, steps, burn, every=1):
@@ -90,7 +107,7 @@ def test_edwards_wilkinson_structure_factor(noise_mode):
 
 
 def test_kpz_structure_factor_same_for_both_noise_draws():
-    n = 64
+    n = 128
     res = []
     for mode in (0, 1):
         sp = spec_kpz(n, nu=1.0, lam=1.0, D=0.05, dt=0.02, seed=21)
@@ -104,7 +121,7 @@ def test_kpz_structure_factor_same_for_both_noise_draws():
         res.append(acc / 1500 / (n * n))
     qx, qy, _ = qgrid(n, n)
     q2 = (qx ** 2 + qy ** 2)[0][:, : n // 2 + 1]
-    fast = q2 > 0.3
+    fast = q2 > 0.5
     r = res[1][fast].sum() / res[0][fast].sum()
     print("  KPZ S(q) spectral / real-space draw: %.4f" % r)
     assert abs(r - 1.0) < 0.01
