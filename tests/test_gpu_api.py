@@ -103,7 +103,14 @@ def test_system_file_model_h(tmp_path):
     o, ids = build_system(O.System, O.pres, sp)
     o.step(50)
     for f in ("phi", "vx", "vy", "w"):
-        err = rel_l2(out(f), o.get_real(ids[f]))
-        print("system file model H", f, "rel L2 vs oracle %.3e" % err)
-        # CSV output carries 6 decimals; vx, vy, w are O(1e-3)
-        assert err < (1e-4 if f == "phi" else 2e-3)
+        got, ref = out(f), o.get_real(ids[f])
+        err = rel_l2(got, ref)
+        dev = float(np.max(np.abs(got - ref)))
+        print("system file model H", f, "rel L2 vs oracle %.3e, max abs %.3e" % (err, dev))
+        if f == "phi":
+            assert err < 1e-4
+        else:
+            # vx, vy, w are O(1e-3) and the CSV carries 6 decimals (half a
+            # unit of the last place = 5e-7): bound the pointwise deviation
+            # by that quantisation plus 1e-4 of the field's scale
+            assert dev <= 5e-7 + 1e-4 * float(np.max(np.abs(ref)))

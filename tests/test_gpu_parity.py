@@ -2,7 +2,7 @@
 FP32 restatement of the reference scheme on the same seeded inputs, plus the
 reference's own golden fixtures and closed forms.
 
-Bar (north star): relative L2 <= 1e-4 after 100 steps against the FP32
+Bar: relative L2 <= 1e-4 after 100 steps against the FP32
 scheme; the FP64 run of the same scheme is reported beside it.
 """
 import numpy as np
