@@ -129,11 +129,49 @@ def bench_ensemble(args, C):
           (len(members), npts / 1e6, args.noise, args.graph), file=sys.stderr, flush=True)
     batch.step(args.warmup)
     batch.synchronize()
+    print(batch.schedule(), file=sys.stderr, end="", flush=True)
     ms = batch.time_steps(args.steps)
     value = npts * args.steps / (ms / 1e3)
     return value, ms, npts, {"plain_value": plain_value, "speedup_vs_plain": value / plain_value,
                              "instances": len(members), "points": npts, "noise": args.noise,
                              "graph_steps": args.graph}
+
+
+def cpu_baseline_ensemble(members, budget_s=20.0):
+    """The oracle (FP32 restatement of the reference CPU scheme, 1 core) on
+    one EW and one KPZ instance of each size, a few steps each; the
+    ensemble's aggregate rate is extrapolated from the per-instance step
+    times (the reference advances one instance at a time)."""
+    from oracle import oracle as O
+    per = {}
+    for n in (64, 128, 256, 512):
+        for kpz in (False, True):
+            lam = 1.0 if kpz else 0.0
+            s, h = kpz_system(O.System, O.pres, n, 1.0, lam, precision="f32")
+            s.step(1)
+            k = max(1, int(budget_s / 8 / (n * n * (2.0e-6 if kpz else 0.7e-6))))
+            k = min(k, 200)
+            t0 = time.perf_counter()
+            s.step(k)
+            per[(n, kpz)] = (time.perf_counter() - t0) / k
+    total_t = sum(per[(n, lam != 0.0)] for n, _, lam in members)
+    npts = sum(n * n for n, _, _ in members)
+    return {"value": npts / total_t, "unit": "grid-point-steps/s", "cores": 1, "kind": "port",
+            "sample": "one EW and one KPZ instance per size (64..512), a few steps each, oracle FP32 "
+                      "(C2C, radix-2); ensemble rate = sum of points / sum of per-instance step times, %s" %
+                      _cpu_name()}
+
+
+def _cpu_name():
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return cpu
 
 
 def cpu_baseline(wl, x0):
@@ -147,15 +185,7 @@ def cpu_baseline(wl, x0):
     s.step(k)
     dt = time.perf_counter() - t0
     n = wl["dims"][0] * wl["dims"][1] * wl["dims"][2]
-    cpu = platform.processor() or platform.machine()
-    try:
-        with open("/proc/cpuinfo") as fh:
-            for line in fh:
-                if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
+    cpu = _cpu_name()
     return {"value": n * k / dt, "unit": "grid-point-steps/s", "cores": 1, "kind": "port",
             "sample": "%d steps of the same %s system after 1 warm-up step, oracle FP32 (C2C, radix-2), %s" %
                       (k, "x".join(map(str, wl["dims"])), cpu)}
@@ -192,6 +222,8 @@ def main():
                "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                             "per": "whole step of the ensemble, 8 B/point algorithmic"}}
+        if not args.no_cpu:
+            out["cpu_baseline"] = cpu_baseline_ensemble(ensemble_members())
         print(json.dumps(out), flush=True)
         return
     dims = wl["dims"]

@@ -73,6 +73,8 @@ _SIGS = {
     "cupss_batch_add": (_I, [_VP, _VP]),
     "cupss_batch_size": (_I, [_VP]),
     "cupss_batch_set_graph_steps": (_I, [_VP, _I]),
+    "cupss_batch_set_autotune": (_I, [_VP, _I]),
+    "cupss_batch_schedule": (_I, [_VP, ctypes.c_char_p, _I]),
     "cupss_batch_advance": (_I, [_VP, _I]),
     "cupss_batch_time_advance": (_I, [_VP, _I, _FP]),
     "cupss_batch_synchronize": (_I, [_VP]),
@@ -257,13 +259,21 @@ class System:
 class Batch:
     """Many systems advanced together by the batched fused step."""
 
-    def __init__(self, graph_steps=0):
+    def __init__(self, graph_steps=0, autotune=True):
         self.L = lib()
         self.h = ctypes.c_void_p()
         check(self.L.cupss_batch_create(None, ctypes.byref(self.h)))
         self.members = []
         if graph_steps:
             check(self.L.cupss_batch_set_graph_steps(self.h, graph_steps))
+        if not autotune:
+            check(self.L.cupss_batch_set_autotune(self.h, 0))
+
+    def schedule(self):
+        """The schedule in use: size classes, tile shapes, chunks."""
+        buf = ctypes.create_string_buffer(4096)
+        check(self.L.cupss_batch_schedule(self.h, buf, len(buf)))
+        return buf.value.decode()
 
     def add(self, system):
         check(self.L.cupss_batch_add(self.h, system.h))

@@ -3,23 +3,33 @@
 // ensembles and parameter sweeps, and (as a batch of one) the fused path of
 // a single system.
 //
-// Instances are grouped into size classes (equal sx/2 and sy).  Per step:
-//   for each class: one row launch, one column launch over the rows /
-//   columns of all its instances (a block table maps each workgroup to an
-//   instance and a first line; consecutive workgroup ids are dealt to the
-//   XCDs so that each XCD sweeps a contiguous range of lines);
-//   one elementwise launch for all instances whose step needs no transform
-//   (no products, noise drawn in spectral space or none).
-// Before the first step of a run a prologue column launch writes the real
-// fields of the current state.  A run of K steps can be captured once in a
-// hipGraph (K even, so the spectra return to their first buffer) and
-// replayed; a small kernel inside the graph advances the step counters
-// that key the noise.
+// Instances are grouped into size classes (equal sx/2 and sy); those whose
+// step needs no transform form the elementwise set.  A run of n steps is
+//   prologue:   tiled transpose of each transform instance's spectrum into
+//               its column-major buffer, column launch (mode 0) writing the
+//               real fields of the current state;
+//   steps:      per class and chunk of instances, K x (row launch, column
+//               launch); one elementwise launch advancing its instances K
+//               steps in registers;
+//   settle:     transpose back, derived fields.
+// Chunks: the instances of a class are split so that the buffers one chunk
+// touches (two spectra, Z_f, X_g, eta) stay resident in the last-level
+// cache for the K steps the chunk takes before the next chunk starts.
+// Instances never interact, so the order changes no result.  K steps can be
+// captured once in a hipGraph (K even, so the spectra return to their first
+// buffer) and replayed; a small kernel inside the graph advances the step
+// counters that key the noise.
+//
+// Tile shapes (rows per row workgroup, columns per column workgroup) and
+// the chunk size are picked per batch by timing the candidates on the
+// batch's own data (autotune); every candidate gives bit-identical results.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <map>
+#include <string>
 #include <vector>
 
 #include "fft_core.hpp"
@@ -36,6 +46,13 @@ __global__ void k_bump_steps(FusedInst *inst, int n, uint32_t by)
     if (i < n) inst[i].step_base += by;
 }
 
+struct Tables {
+    FusedBlock *rb = nullptr;
+    int nrb = 0;
+    FusedBlock *cb = nullptr;
+    int ncb = 0;
+};
+
 struct FusedBatch {
     std::vector<cupss_system> sys;
     hipStream_t stream = nullptr;
@@ -43,20 +60,18 @@ struct FusedBatch {
 
     struct Class {
         int nc = 0, ny = 0;
-        bool rows = false;      // any instance needs the row pass
         bool real = false;      // any instance has real fields (prologue)
         std::vector<int> insts;
-        FusedBlock *rb = nullptr;
-        int nrb = 0;
-        FusedBlock *cb = nullptr;
-        int ncb = 0;
+        int lpb = 0, twd = 0;   // tile shapes
+        std::vector<Tables> chunks;
     };
     std::vector<Class> classes;
     std::vector<int> elem;      // elementwise instances
-    FusedBlock *eb = nullptr;
+    std::vector<char> is_elem;  // per instance
+    FusedBlock *eb = nullptr;   // elementwise: FUSED_ELEM_BLOCK bins per block
     int neb = 0;
-    FusedBlock *db = nullptr;   // rows of instances with derived fields
-    int ndb = 0;
+    FusedBlock *tb = nullptr;   // transpose tiles of the transform instances
+    int ntb = 0;
     hipEvent_t done = nullptr;  // recorded after each run; member streams wait on it
 
     std::vector<FusedInst> h;
@@ -64,6 +79,10 @@ struct FusedBatch {
     size_t dcap = 0;
     FusedInst *pinned = nullptr;  // staging of the descriptor upload
     bool built = false;
+
+    bool autotune = true;
+    bool tuned = false;
+    double chunk_bytes = 0;       // chunk working-set target (0: one chunk per class)
 
     int graph_steps = 0;
     hipGraphExec_t gexec = nullptr;
@@ -74,25 +93,38 @@ struct FusedBatch {
 
     ~FusedBatch() { release(); }
 
-    void release_tables()
+    void drop_graph()
     {
-        for (auto &c : classes) {
-            hipFree(c.rb);
-            hipFree(c.cb);
-        }
-        classes.clear();
-        hipFree(eb);
-        eb = nullptr;
-        neb = 0;
-        hipFree(db);
-        db = nullptr;
-        ndb = 0;
-        elem.clear();
         if (gexec) hipGraphExecDestroy(gexec);
         if (graph) hipGraphDestroy(graph);
         gexec = nullptr;
         graph = nullptr;
+    }
+
+    static void free_tables(std::vector<Tables> &v)
+    {
+        for (auto &t : v) {
+            hipFree(t.rb);
+            hipFree(t.cb);
+        }
+        v.clear();
+    }
+
+    void release_tables()
+    {
+        for (auto &c : classes) free_tables(c.chunks);
+        classes.clear();
+        hipFree(eb);
+        eb = nullptr;
+        neb = 0;
+        hipFree(tb);
+        tb = nullptr;
+        ntb = 0;
+        elem.clear();
+        is_elem.clear();
+        drop_graph();
         built = false;
+        tuned = false;
     }
 
     void release()
@@ -111,6 +143,14 @@ struct FusedBatch {
     }
 };
 
+static bool is_elementwise(const FusedInst &a) { return a.ngroups == 0 && a.nreal == 0 && a.noise_mode != NOISE_REAL; }
+
+static int alloc_spec(float2 **p, const Geom &g)
+{
+    if (!*p && hipMalloc(p, sizeof(float2) * g.nh) != hipSuccess) return fail(CUPSS_ERR_NOMEM, "device allocation failed");
+    return CUPSS_OK;
+}
+
 // device descriptor of system s (its fused plan must be ok)
 static int describe(cupss_system s, FusedInst &a)
 {
@@ -118,15 +158,11 @@ static int describe(cupss_system s, FusedInst &a)
     const Geom &g = s->g;
     FieldRec &F = s->f[P.phi];
     for (size_t i = 0; i < P.real.size(); i++)
-        if (!s->fz[i] && hipMalloc(&s->fz[i], sizeof(float2) * g.nh) != hipSuccess)
-            return fail(CUPSS_ERR_NOMEM, "device allocation failed");
+        if (int r = alloc_spec(&s->fz[i], g)) return r;
     for (size_t i = 0; i < P.groups.size(); i++)
-        if (!s->fx[i] && hipMalloc(&s->fx[i], sizeof(float2) * g.nh) != hipSuccess)
-            return fail(CUPSS_ERR_NOMEM, "device allocation failed");
+        if (int r = alloc_spec(&s->fx[i], g)) return r;
     std::memset(&a, 0, sizeof(a));
     a.g = g;
-    a.spec[0] = F.spec[F.cur];
-    a.spec[1] = F.spec[F.cur ^ 1];
     a.eta = F.eta;
     a.step_base = s->step;
     build_update(s, P.phi, a.u);
@@ -146,6 +182,7 @@ static int describe(cupss_system s, FusedInst &a)
     for (size_t i = 0; i < P.real.size(); i++) {
         const int f = P.real[i];
         a.p[i] = s->f[f].order;
+        a.kmax[i] = a.p[i] > 0 ? std::min(g.nc, g.sx / (a.p[i] + 1)) : g.nc;
         a.Z[i] = s->fz[i];
         a.zero[i] = (f == P.phi && F.dstate == 0) ? 1 : 0;
         if (f != P.phi) {
@@ -187,7 +224,6 @@ static int describe(cupss_system s, FusedInst &a)
         const FieldRec &D = s->f[P.derived[d]];
         const TermRec &T = s->t[D.terms[0]];
         const int m = (T.pres[0].iqx + T.pres[0].iqy + T.pres[0].iqz) % 2;
-        a.dout[d] = D.spec[D.cur];
         a.dmul[d].mult_i = m;
         a.dmul[d].pres0 = dp;
         a.dmul[d].npres = (int)T.pres.size();
@@ -206,10 +242,33 @@ static int describe(cupss_system s, FusedInst &a)
     a.key0 = (uint32_t)F.seed;
     a.key1 = (uint32_t)(F.seed >> 32);
     a.stream = (uint32_t)P.phi;
+    if (!is_elementwise(a)) {
+        if (int r = alloc_spec(&s->ft[0], g)) return r;
+        if (int r = alloc_spec(&s->ft[1], g)) return r;
+    }
     return CUPSS_OK;
 }
 
-static bool is_elementwise(const FusedInst &a) { return a.ngroups == 0 && a.nreal == 0 && a.noise_mode != NOISE_REAL; }
+// the per-run parts of a descriptor: buffers, step counter, step-0 flags
+static void run_view(cupss_system s, bool elementwise, FusedInst &a)
+{
+    FieldRec &F = s->f[s->plan.phi];
+    if (elementwise) {
+        a.spec[0] = F.spec[F.cur];
+        a.spec[1] = F.spec[F.cur ^ 1];
+        a.nat = nullptr;
+    } else {
+        a.spec[0] = s->ft[0];
+        a.spec[1] = s->ft[1];
+        a.nat = F.spec[F.cur];
+    }
+    a.step_base = s->step;
+    for (int k = 0; k < a.nreal; k++) a.zero[k] = (s->plan.real[k] == s->plan.phi && F.dstate == 0) ? 1 : 0;
+    for (int d = 0; d < a.nder; d++) {
+        FieldRec &D = s->f[s->plan.derived[d]];
+        a.dout[d] = D.spec[D.cur];
+    }
+}
 
 // table of (instance, first line) items in XCD-aware order
 static int upload_blocks(const std::vector<FusedBlock> &items, FusedBlock **dst, int *n)
@@ -226,6 +285,56 @@ static int upload_blocks(const std::vector<FusedBlock> &items, FusedBlock **dst,
         return fail(CUPSS_ERR_HIP, "hipMemcpy failed");
     *n = (int)nb;
     return CUPSS_OK;
+}
+
+// bytes of the pipeline buffers one transform instance touches per step
+static double inst_bytes(const FusedInst &a)
+{
+    const int arrays = 2 + a.nreal + a.ngroups + (a.noise_mode == NOISE_REAL ? 1 : 0);
+    return 8.0 * (double)a.g.nh * arrays;
+}
+
+// row and column tables of the chunks of class c for shapes (lpb, twd)
+static int class_tables(const FusedBatch &B, const FusedBatch::Class &c, int lpb, int twd, double chunk_bytes,
+                        std::vector<Tables> &out)
+{
+    std::vector<std::vector<int>> parts;
+    double acc = 0;
+    for (int i : c.insts) {
+        const double b = inst_bytes(B.h[i]);
+        if (parts.empty() || (chunk_bytes > 0 && acc + b > chunk_bytes && acc > 0)) {
+            parts.emplace_back();
+            acc = 0;
+        }
+        parts.back().push_back(i);
+        acc += b;
+    }
+    for (auto &part : parts) {
+        std::vector<FusedBlock> rows, cols;
+        for (int i : part) {
+            const Geom &g = B.h[i].g;
+            for (int r0 = 0; r0 < g.sy; r0 += lpb) rows.push_back(FusedBlock{i, r0});
+            for (int c0 = 0; c0 < g.nc1; c0 += twd) cols.push_back(FusedBlock{i, c0});
+        }
+        out.push_back(Tables());
+        if (int r = upload_blocks(rows, &out.back().rb, &out.back().nrb)) return r;
+        if (int r = upload_blocks(cols, &out.back().cb, &out.back().ncb)) return r;
+    }
+    return CUPSS_OK;
+}
+
+// shapes before (or without) tuning: up to 8 rows per row workgroup, up to
+// 256 threads per column workgroup
+static void default_shapes(FusedBatch::Class &c)
+{
+    const std::vector<int> rs = fused_row_shapes(c.nc, c.ny), cs = fused_col_shapes(c.ny);
+    c.lpb = rs.empty() ? 0 : rs.front();
+    for (int v : rs)
+        if (v <= 8) c.lpb = v;
+    c.twd = cs.empty() ? 0 : cs.front();
+    const int T = c.ny >= 4096 ? c.ny / 16 : (c.ny >= 8 ? c.ny / 8 : 1);
+    for (int v : cs)
+        if (v * T <= 256) c.twd = v;
 }
 
 static int build(FusedBatch &B)
@@ -247,10 +356,14 @@ static int build(FusedBatch &B)
         B.dcap = B.h.size();
     }
     std::map<std::pair<int, int>, int> cls;
+    B.is_elem.assign(B.h.size(), 0);
+    std::vector<FusedBlock> tiles, er;
     for (size_t i = 0; i < B.h.size(); i++) {
         const FusedInst &a = B.h[i];
         if (is_elementwise(a)) {
             B.elem.push_back((int)i);
+            B.is_elem[i] = 1;
+            for (long h0 = 0; h0 < a.g.nh; h0 += FUSED_ELEM_BLOCK) er.push_back(FusedBlock{(int)i, (int)h0});
             continue;
         }
         auto key = std::make_pair(a.g.nc, a.g.sy);
@@ -262,31 +375,17 @@ static int build(FusedBatch &B)
         }
         FusedBatch::Class &c = B.classes[cls[key]];
         c.insts.push_back((int)i);
-        c.rows = true;
         c.real = c.real || a.nreal > 0;
+        const int ntx = (a.g.nc1 + FUSED_TILE - 1) / FUSED_TILE, nty = (a.g.sy + FUSED_TILE - 1) / FUSED_TILE;
+        for (int q = 0; q < ntx * nty; q++) tiles.push_back(FusedBlock{(int)i, q});
     }
     for (auto &c : B.classes) {
-        std::vector<FusedBlock> rows, cols;
-        const int lpb = fused_rows_per_block(c.nc);
-        const int twd = fused_cols_per_block(c.ny);
-        for (int i : c.insts) {
-            const Geom &g = B.h[i].g;
-            for (int r0 = 0; r0 < g.sy * g.sz; r0 += lpb) rows.push_back(FusedBlock{i, r0});
-            for (int c0 = 0; c0 < g.nc1; c0 += twd) cols.push_back(FusedBlock{i, c0});
-        }
-        if (int r = upload_blocks(rows, &c.rb, &c.nrb)) return r;
-        if (int r = upload_blocks(cols, &c.cb, &c.ncb)) return r;
+        default_shapes(c);
+        if (c.lpb <= 0 || c.twd <= 0) return fail(CUPSS_ERR_UNSUPPORTED, "no tile shape for this size class");
+        if (int r = class_tables(B, c, c.lpb, c.twd, B.chunk_bytes, c.chunks)) return r;
     }
-    std::vector<FusedBlock> er, dr;
-    for (int i : B.elem) {
-        const Geom &g = B.h[i].g;
-        for (int r0 = 0; r0 < g.sy * g.sz; r0++) er.push_back(FusedBlock{i, r0});
-    }
-    for (size_t i = 0; i < B.h.size(); i++)
-        if (B.h[i].nder > 0)
-            for (int r0 = 0; r0 < B.h[i].g.sy * B.h[i].g.sz; r0++) dr.push_back(FusedBlock{(int)i, r0});
     if (int r = upload_blocks(er, &B.eb, &B.neb)) return r;
-    if (int r = upload_blocks(dr, &B.db, &B.ndb)) return r;
+    if (int r = upload_blocks(tiles, &B.tb, &B.ntb)) return r;
     B.gens.clear();
     for (cupss_system s : B.sys) B.gens.push_back(s->gen);
     B.built = true;
@@ -301,25 +400,12 @@ static bool stale_tables(const FusedBatch &B)
     return false;
 }
 
-// refresh the per-run parts of the descriptors (buffers, step counters,
-// step-0 flags) and upload them; the previous run must be finished with the
-// device table and the staging buffer before either is rewritten
+// upload the per-run descriptors; the previous run must be finished with
+// the device table and the staging buffer before either is rewritten
 static int refresh(FusedBatch &B)
 {
     HIP_TRY(hipStreamSynchronize(B.stream));
-    for (size_t i = 0; i < B.sys.size(); i++) {
-        cupss_system s = B.sys[i];
-        FieldRec &F = s->f[s->plan.phi];
-        FusedInst &a = B.h[i];
-        a.spec[0] = F.spec[F.cur];
-        a.spec[1] = F.spec[F.cur ^ 1];
-        a.step_base = s->step;
-        for (int k = 0; k < a.nreal; k++) a.zero[k] = (s->plan.real[k] == s->plan.phi && F.dstate == 0) ? 1 : 0;
-        for (int d = 0; d < a.nder; d++) {
-            FieldRec &D = s->f[s->plan.derived[d]];
-            a.dout[d] = D.spec[D.cur];
-        }
-    }
+    for (size_t i = 0; i < B.sys.size(); i++) run_view(B.sys[i], B.is_elem[i], B.h[i]);
     std::memcpy(B.pinned, B.h.data(), sizeof(FusedInst) * B.h.size());
     HIP_TRY(hipMemcpyAsync(B.d, B.pinned, sizeof(FusedInst) * B.h.size(), hipMemcpyHostToDevice, B.stream));
     return CUPSS_OK;
@@ -327,34 +413,157 @@ static int refresh(FusedBatch &B)
 
 static int launch_prologue(FusedBatch &B)
 {
+    if (B.ntb > 0) HIP_TRY(fused_transpose_in_launch(B.d, B.tb, B.ntb, B.stream));
     for (auto &c : B.classes)
-        if (c.real) HIP_TRY(fused_cols_launch(c.ny, B.d, c.cb, c.ncb, 0, 0, 0, B.stream));
+        if (c.real)
+            for (auto &t : c.chunks)
+                HIP_TRY(fused_cols_launch(c.ny, c.twd, B.d, t.cb, t.ncb, 0, 0, 0, B.stream));
     return CUPSS_OK;
 }
 
-static int launch_step(FusedBatch &B, int parity, uint32_t rel)
+// steps [n0, n0+k) of every transform instance, chunk by chunk; step n
+// reads spectrum buffer n&1 and uses counter offset n - rel0
+static int launch_steps(FusedBatch &B, int n0, int k, int rel0)
 {
-    for (auto &c : B.classes) {
-        if (c.rows) HIP_TRY(fused_rows_launch(c.nc, B.d, c.rb, c.nrb, rel, B.stream));
-        HIP_TRY(fused_cols_launch(c.ny, B.d, c.cb, c.ncb, parity, rel, 1, B.stream));
-    }
-    if (B.neb > 0) HIP_TRY(fused_elem_launch(B.d, B.eb, B.neb, parity, rel, B.stream));
+    for (auto &c : B.classes)
+        for (auto &t : c.chunks)
+            for (int n = n0; n < n0 + k; n++) {
+                HIP_TRY(fused_rows_launch(c.nc, c.lpb, B.d, t.rb, t.nrb, (uint32_t)(n - rel0), B.stream));
+                HIP_TRY(fused_cols_launch(c.ny, c.twd, B.d, t.cb, t.ncb, n & 1, (uint32_t)(n - rel0), 1, B.stream));
+            }
     return CUPSS_OK;
 }
 
-// after a run: the general path's view of every system.  Each derived field
-// holds P(M_psi phi) of the last step's starting phi, i.e. of the spectrum
-// buffer the last step read (parity (nsteps-1) & 1); all of them are
-// written by one launch.  Member streams then wait for the batch stream.
+// ---- autotune
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~Timer()
+    {
+        if (a) hipEventDestroy(a);
+        if (b) hipEventDestroy(b);
+    }
+};
+
+template <class F>
+static int time_ms(FusedBatch &B, Timer &T, int reps, F &&launch, float *ms)
+{
+    if (!T.a) {
+        HIP_TRY(hipEventCreate(&T.a));
+        HIP_TRY(hipEventCreate(&T.b));
+    }
+    if (int r = launch()) return r;  // warm-up
+    HIP_TRY(hipEventRecord(T.a, B.stream));
+    for (int i = 0; i < reps; i++)
+        if (int r = launch()) return r;
+    HIP_TRY(hipEventRecord(T.b, B.stream));
+    HIP_TRY(hipEventSynchronize(T.b));
+    HIP_TRY(hipEventElapsedTime(ms, T.a, T.b));
+    return CUPSS_OK;
+}
+
+// Time every tile shape of every class on the batch's own data, then the
+// chunk sizes, and keep the fastest.  Runs between a prologue and the
+// restoring prologue that follows it: the launches only rewrite buffers
+// that the prologue and the steps rewrite anyway.
+static int tune(FusedBatch &B)
+{
+    Timer T;
+    const int reps = 3;
+    for (auto &c : B.classes) {
+        // rows
+        float best = 1e30f;
+        int pick = c.lpb;
+        for (int lpb : fused_row_shapes(c.nc, c.ny)) {
+            std::vector<Tables> t;
+            if (int r = class_tables(B, c, lpb, c.twd, 0, t)) return r;
+            float ms = 0;
+            const int r = time_ms(
+                B, T, reps,
+                [&]() -> int {
+                    HIP_TRY(fused_rows_launch(c.nc, lpb, B.d, t[0].rb, t[0].nrb, 0, B.stream));
+                    return CUPSS_OK;
+                },
+                &ms);
+            HIP_TRY(hipStreamSynchronize(B.stream));
+            FusedBatch::free_tables(t);
+            if (r) return r;
+            if (ms < best) {
+                best = ms;
+                pick = lpb;
+            }
+        }
+        c.lpb = pick;
+        // columns
+        best = 1e30f;
+        pick = c.twd;
+        for (int twd : fused_col_shapes(c.ny)) {
+            std::vector<Tables> t;
+            if (int r = class_tables(B, c, c.lpb, twd, 0, t)) return r;
+            float ms = 0;
+            const int r = time_ms(
+                B, T, reps,
+                [&]() -> int {
+                    HIP_TRY(fused_cols_launch(c.ny, twd, B.d, t[0].cb, t[0].ncb, 0, 0, 1, B.stream));
+                    return CUPSS_OK;
+                },
+                &ms);
+            HIP_TRY(hipStreamSynchronize(B.stream));
+            FusedBatch::free_tables(t);
+            if (r) return r;
+            if (ms < best) {
+                best = ms;
+                pick = twd;
+            }
+        }
+        c.twd = pick;
+    }
+    // chunk size: whole classes, or working sets of 128 / 64 / 32 MB
+    double total = 0;
+    for (auto &c : B.classes)
+        for (int i : c.insts) total += inst_bytes(B.h[i]);
+    std::vector<double> cand = {0};
+    for (double mb : {128.0, 64.0, 32.0})
+        if (total > mb * 1048576.0) cand.push_back(mb * 1048576.0);
+    float best = 1e30f;
+    double pick = 0;
+    for (double cb : cand) {
+        HIP_TRY(hipStreamSynchronize(B.stream));
+        for (auto &c : B.classes) {
+            FusedBatch::free_tables(c.chunks);
+            if (int r = class_tables(B, c, c.lpb, c.twd, cb, c.chunks)) return r;
+        }
+        float ms = 0;
+        if (int r = time_ms(B, T, 1, [&]() { return launch_steps(B, 0, 4, 0); }, &ms)) return r;
+        if (ms < best) {
+            best = ms;
+            pick = cb;
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(B.stream));
+    B.chunk_bytes = pick;
+    for (auto &c : B.classes) {
+        FusedBatch::free_tables(c.chunks);
+        if (int r = class_tables(B, c, c.lpb, c.twd, B.chunk_bytes, c.chunks)) return r;
+    }
+    B.tuned = true;
+    return CUPSS_OK;
+}
+
+// after a run: the general path's view of every system.  Transform
+// instances get their spectrum back in natural layout and each derived
+// field P(M_psi phi) of the last step's starting phi; elementwise instances
+// wrote both in place.  Member streams then wait for the batch stream.
 static int settle(FusedBatch &B, int nsteps)
 {
-    if (B.ndb > 0) HIP_TRY(fused_derived_launch(B.d, B.db, B.ndb, (nsteps - 1) & 1, B.stream));
+    if (B.ntb > 0) HIP_TRY(fused_settle_launch(B.d, B.tb, B.ntb, nsteps & 1, B.stream));
     if (!B.done) HIP_TRY(hipEventCreateWithFlags(&B.done, hipEventDisableTiming));
     HIP_TRY(hipEventRecord(B.done, B.stream));
-    for (cupss_system s : B.sys) {
+    for (size_t i = 0; i < B.sys.size(); i++) {
+        cupss_system s = B.sys[i];
         const FusedPlan &P = s->plan;
         FieldRec &F = s->f[P.phi];
-        F.cur ^= (nsteps & 1);
+        if (B.is_elem[i]) F.cur ^= (nsteps & 1);
         s->step += (uint64_t)nsteps;
         s->fused_steps += nsteps;
         if (F.order > 0) F.dstate = 2;
@@ -375,13 +584,21 @@ static int run(FusedBatch &B, int nsteps)
     if (int r = refresh(B)) return r;
     if (!twiddle_sets(B.stream)) return fail(CUPSS_ERR_NOMEM, "twiddle tables");
     if (int r = launch_prologue(B)) return r;
+    if (B.autotune && !B.tuned && !B.classes.empty()) {
+        B.drop_graph();
+        if (int r = tune(B)) return r;
+        if (int r = launch_prologue(B)) return r;
+    }
     int done = 0;
     const int K = B.graph_steps;
     if (K >= 2 && nsteps >= K) {
         if (!B.gexec) {
             HIP_TRY(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
-            int rc = CUPSS_OK;
-            for (int n = 0; n < K && rc == CUPSS_OK; n++) rc = launch_step(B, n & 1, (uint32_t)n);
+            int rc = launch_steps(B, 0, K, 0);
+            if (rc == CUPSS_OK && B.neb > 0) {
+                const hipError_t e = fused_elem_launch(B.d, B.eb, B.neb, 0, 0, K, B.stream);
+                if (e != hipSuccess) rc = fail(CUPSS_ERR_HIP, hipGetErrorString(e));
+            }
             if (rc == CUPSS_OK) {
                 const int ni = (int)B.h.size();
                 hipLaunchKernelGGL(k_bump_steps, dim3((ni + 255) / 256), dim3(256), 0, B.stream, B.d, ni,
@@ -403,8 +620,10 @@ static int run(FusedBatch &B, int nsteps)
         // the remainder continues from the bumped counters
         for (size_t i = 0; i < B.h.size(); i++) B.h[i].step_base += (uint64_t)done;
     }
-    for (int n = done; n < nsteps; n++)
-        if (int r = launch_step(B, n & 1, (uint32_t)(n - done))) return r;
+    if (nsteps > done) {
+        if (int r = launch_steps(B, done, nsteps - done, done)) return r;
+        if (B.neb > 0) HIP_TRY(fused_elem_launch(B.d, B.eb, B.neb, done & 1, 0, nsteps - done, B.stream));
+    }
     B.steps_run += nsteps;
     return settle(B, nsteps);
 }
@@ -429,6 +648,21 @@ void fused_release(cupss_system s)
 {
     delete static_cast<FusedBatch *>(s->batch);
     s->batch = nullptr;
+}
+
+static std::string schedule_text(const FusedBatch &B)
+{
+    std::string out;
+    char line[256];
+    for (const auto &c : B.classes) {
+        std::snprintf(line, sizeof line, "class %dx%d: %zu instances, rows/wg %d, cols/wg %d, chunks %zu\n",
+                      2 * c.nc, c.ny, c.insts.size(), c.lpb, c.twd, c.chunks.size());
+        out += line;
+    }
+    std::snprintf(line, sizeof line, "elementwise: %zu instances\nchunk target: %.0f MB%s\n", B.elem.size(),
+                  B.chunk_bytes / 1048576.0, B.tuned ? " (autotuned)" : "");
+    out += line;
+    return out;
 }
 
 }  // namespace cupss
@@ -486,7 +720,23 @@ extern "C" int cupss_batch_set_graph_steps(cupss_batch b, int k)
     if (!b) return fail(CUPSS_ERR_INVALID, "null batch");
     if (k != 0 && (k < 2 || (k & 1))) return fail(CUPSS_ERR_INVALID, "graph steps must be 0 or an even number >= 2");
     b->b.graph_steps = k;
+    b->b.drop_graph();
+    return CUPSS_OK;
+}
+
+extern "C" int cupss_batch_set_autotune(cupss_batch b, int on)
+{
+    if (!b) return fail(CUPSS_ERR_INVALID, "null batch");
+    b->b.autotune = on != 0;
     b->b.built = false;
+    return CUPSS_OK;
+}
+
+extern "C" int cupss_batch_schedule(cupss_batch b, char *buf, int len)
+{
+    if (!b || !buf || len <= 0) return fail(CUPSS_ERR_INVALID, "null batch or buffer");
+    const std::string t = schedule_text(b->b);
+    std::snprintf(buf, (size_t)len, "%s", t.c_str());
     return CUPSS_OK;
 }
 

@@ -5,14 +5,19 @@
 // serves any number of instances (systems) of one size class through a
 // table of instance descriptors (FusedInst) and a per-workgroup block table.
 //
-// Pipeline state between the kernels (natural half-spectrum layout):
+// Pipeline state between the kernels, all column-major (element (kx, y) at
+// kx*sy + y) so that the column kernel streams whole contiguous columns:
+//   phi_hat  two spectra, [parity] in and [parity^1] out
 //   Z_f  = IFFT_y(mask_f * L_f * phi_hat)   for each real field f needed by a
-//          product (f = phi, L = 1; f = psi, L = M_psi)
+//          product (f = phi, L = 1; f = psi, L = M_psi); only the columns
+//          kx <= sx/(p_f+1) that the dealias mask keeps are stored
 //   X_g  = R2C_x(sum of the products of group g)  (rows transformed only)
 //   eta  = R2C_x(xi)                        (real-space noise, rows only)
 // Per step:
-//   row kernel:    Z_f rows -C2R-> real dealiased fields -> products,
-//                  noise -R2C-> X_g, eta rows
+//   row kernel:    a tile of LPB rows of each Z_f is staged through LDS
+//                  (runs of LPB*8 contiguous bytes per column), C2R along x
+//                  -> real dealiased fields -> products, noise -R2C-> X_g,
+//                  eta rows, written back through the same tile
 //   column kernel: X_g, eta columns -FFT_y-> spectral terms; update of
 //                  phi (setDynamic_k); mask_f * L_f * phi_new -IFFT_y-> Z_f
 // Terms with equal prefactor lists form one group: M*FFT(a) + M*FFT(b) =
@@ -21,9 +26,10 @@
 // at the start of the step (phase A/B) and used by phi's products in phase
 // C; phi's own dealiased copy is zero before its first update (step 0).
 //
-// Each workgroup copies the twiddle table of its transform length into LDS
-// and issues its global loads before the first transform, so that their
-// latencies overlap.
+// A run starts with a tiled transpose of the system's spectrum into the
+// column-major buffer and ends with the transpose back (plus the derived
+// fields); instances whose step needs no transform advance any number of
+// steps per launch in registers (k_felem).
 #include <hip/hip_runtime.h>
 
 #include "fft_core.hpp"
@@ -35,24 +41,52 @@ namespace cupss {
 
 // ------------------------------------------------------------ row kernel
 
-template <int NC>
+template <int NC, int LPB>
 struct FRowCfg {
-    static constexpr int E = row_elems(NC);
+    static constexpr int E = NC >= 1024 ? 16 : (NC >= 64 ? 8 : (NC >= 4 ? 4 : NC));
     static constexpr int T = NC / E;
-    static constexpr int LPB = T >= 64 ? 1 : 64 / T;
     static constexpr int THREADS = LPB * T;
-    static constexpr int LL = lds_len(NC);
+    static constexpr int RS = lds_len(NC);  // tile row stride (>= NC + 1), odd for NC >= 32
+    static constexpr bool valid()
+    {
+        return THREADS >= 16 && THREADS <= 512 && (LPB * RS + 2 * NC) * 8 <= 150 * 1024;
+    }
 };
 
-template <int NC, int E, int T>
-__device__ __forceinline__ void load_half_row(const float2 *__restrict__ x, bool active, int t, float2 (&xa)[E],
-                                              float2 (&ya)[E])
+// rows y0 .. y0+LPB-1 of a column-major half spectrum into the tile (row l
+// at l*RS, entry k at k); columns k > kmax read as zero.  Pairs of rows are
+// moved with one 16-byte access.
+template <int NC, int LPB, int RS, int TH>
+__device__ __forceinline__ void tile_load(float2 *__restrict__ tile, const float2 *__restrict__ src, int sy, int y0,
+                                          int kmax)
 {
-#pragma unroll
-    for (int e = 0; e < E; e++) {
-        const int k = t + e * T;
-        xa[e] = active ? x[k] : make_float2(0.f, 0.f);
-        ya[e] = active ? x[NC - k] : make_float2(0.f, 0.f);
+    if constexpr (LPB >= 2) {
+        constexpr int HP = LPB / 2;
+        for (int i = threadIdx.x; i < (NC + 1) * HP; i += TH) {
+            const int k = i / HP, yl = 2 * (i % HP);
+            float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (k <= kmax) w = *reinterpret_cast<const float4 *>(src + (long)k * sy + y0 + yl);
+            tile[yl * RS + k] = make_float2(w.x, w.y);
+            tile[(yl + 1) * RS + k] = make_float2(w.z, w.w);
+        }
+    } else {
+        for (int k = threadIdx.x; k <= NC; k += TH)
+            tile[k] = k <= kmax ? src[(long)k * sy + y0] : make_float2(0.f, 0.f);
+    }
+}
+
+template <int NC, int LPB, int RS, int TH>
+__device__ __forceinline__ void tile_store(float2 *__restrict__ dst, const float2 *__restrict__ tile, int sy, int y0)
+{
+    if constexpr (LPB >= 2) {
+        constexpr int HP = LPB / 2;
+        for (int i = threadIdx.x; i < (NC + 1) * HP; i += TH) {
+            const int k = i / HP, yl = 2 * (i % HP);
+            const float2 a = tile[yl * RS + k], b = tile[(yl + 1) * RS + k];
+            *reinterpret_cast<float4 *>(dst + (long)k * sy + y0 + yl) = make_float4(a.x, a.y, b.x, b.y);
+        }
+    } else {
+        for (int k = threadIdx.x; k <= NC; k += TH) dst[(long)k * sy + y0] = tile[k];
     }
 }
 
@@ -77,68 +111,71 @@ __device__ __forceinline__ void c2r_prep(const float2 (&xa)[E], const float2 (&y
     }
 }
 
-// R2C of one row whose real values are packed as v[e] = (x[2m], x[2m+1])
+// R2C of one row whose real values are packed as v[e] = (x[2m], x[2m+1]);
+// the NC+1 bins are left in L[0..NC] (unpadded).  Ends with a barrier.
 template <int NC, int E, int T>
-__device__ __forceinline__ void r2c_store(float2 (&v)[E], bool active, float2 *__restrict__ o, float2 *L, int t,
-                                          const float2 *tws)
+__device__ __forceinline__ void r2c_tile(float2 (&v)[E], float2 *L, int t, const float2 *tws)
 {
     fft_line<NC, E, -1, 2 * NC>(v, L, t, tws);
-    if (active) {
+    float2 o[E];
 #pragma unroll
-        for (int e = 0; e < E; e++) {
-            const int k = t + e * T;
-            const float2 A = v[e];
-            const float2 B = cconj(L[lpad((NC - k) & (NC - 1))]);
-            const float2 xe = cscale(cadd(A, B), 0.5f);
-            const float2 xo = cscale(mul_i<-1>(csub(A, B)), 0.5f);
-            o[k] = k == 0 ? make_float2(xe.x + xo.x, 0.0f)
-                          : cadd(xe, cmul(tw_lookup<-1, 2 * NC>(tws, k, 2 * NC), xo));
-        }
-        if (t == 0) {
-            const float2 z0 = L[lpad(0)];
-            o[NC] = make_float2(z0.x - z0.y, 0.0f);
-        }
+    for (int e = 0; e < E; e++) {
+        const int k = t + e * T;
+        const float2 A = v[e];
+        const float2 B = cconj(L[lpad((NC - k) & (NC - 1))]);
+        const float2 xe = cscale(cadd(A, B), 0.5f);
+        const float2 xo = cscale(mul_i<-1>(csub(A, B)), 0.5f);
+        o[e] = k == 0 ? make_float2(xe.x + xo.x, 0.0f) : cadd(xe, cmul(tw_lookup<-1, 2 * NC>(tws, k, 2 * NC), xo));
     }
-    __syncthreads();  // the LDS line is reused by the next transform
+    const float2 z0 = L[lpad(0)];
+    __syncthreads();  // every read of the transform's LDS line is done
+#pragma unroll
+    for (int e = 0; e < E; e++) L[t + e * T] = o[e];
+    if (t == 0) L[NC] = make_float2(z0.x - z0.y, 0.0f);
+    __syncthreads();
 }
 
-template <int NC>
-__global__ __launch_bounds__(256) void k_frows(const FusedInst *__restrict__ insts,
-                                               const FusedBlock *__restrict__ blocks, uint32_t step_rel,
-                                               const float2 *__restrict__ sets)
+template <int NC, int LPB>
+__global__ __launch_bounds__((FRowCfg<NC, LPB>::THREADS)) void k_frows(const FusedInst *__restrict__ insts,
+                                                                    const FusedBlock *__restrict__ blocks,
+                                                                    uint32_t step_rel,
+                                                                    const float2 *__restrict__ sets)
 {
-    using C = FRowCfg<NC>;
-    constexpr int E = C::E, T = C::T;
+    using C = FRowCfg<NC, LPB>;
+    constexpr int E = C::E, T = C::T, RS = C::RS, TH = C::THREADS;
     __shared__ float2 tws[2 * NC];
-    __shared__ float2 lds[C::LPB * C::LL];
+    __shared__ float2 tile[LPB * RS];
     const FusedBlock blk = blocks[blockIdx.x];
     if (blk.inst < 0) return;
     const FusedInst &a = insts[blk.inst];
+    const int sy = a.g.sy, y0 = blk.line;
     const int l = threadIdx.x / T, t = threadIdx.x % T;
-    const long row = (long)blk.line + l;
-    const bool active = row < (long)a.g.sy * a.g.sz;
-    float2 *L = lds + l * C::LL;
-    const long hrow = row * (NC + 1);
+    float2 *L = tile + l * RS;
     const int nreal = a.nreal;
-
-    float2 xa[E], ya[E];
-    if (nreal > 0) load_half_row<NC, E, T>(a.Z[0] + hrow, active, t, xa, ya);
     load_twiddles<2 * NC>(tws, sets);
-    __syncthreads();
 
     float2 r[FUSED_MAX_REAL][E];
 #pragma unroll
     for (int f = 0; f < FUSED_MAX_REAL; f++) {
         if (f < nreal) {
-            float2 v[E];
+            tile_load<NC, LPB, RS, TH>(tile, a.Z[f], sy, y0, a.kmax[f]);
+            __syncthreads();
+            float2 xa[E], ya[E], v[E];
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int k = t + e * T;
+                xa[e] = L[k];
+                ya[e] = L[NC - k];
+            }
+            __syncthreads();  // the tile becomes the transform's LDS
             c2r_prep<NC, E, T>(xa, ya, t, v, tws);
-            if (f + 1 < nreal) load_half_row<NC, E, T>(a.Z[f + 1] + hrow, active, t, xa, ya);
             fft_line<NC, E, +1, 2 * NC>(v, L, t, tws);
 #pragma unroll
             for (int e = 0; e < E; e++) r[f][e] = cscale(v[e], a.scale);
             __syncthreads();
         }
     }
+    if (nreal == 0) __syncthreads();  // twiddles
     for (int gi = 0; gi < a.ngroups; gi++) {
         float2 v[E];
 #pragma unroll
@@ -160,37 +197,41 @@ __global__ __launch_bounds__(256) void k_frows(const FusedInst *__restrict__ ins
 #pragma unroll
             for (int e = 0; e < E; e++) v[e] = cadd(v[e], p[e]);
         }
-        r2c_store<NC, E, T>(v, active, a.X[gi] + hrow, L, t, tws);
+        r2c_tile<NC, E, T>(v, L, t, tws);
+        tile_store<NC, LPB, RS, TH>(a.X[gi], tile, sy, y0);
+        __syncthreads();
     }
     if (a.noise_mode == NOISE_REAL) {
         const uint64_t step = a.step_base + step_rel;
         float2 v[E];
-        const uint32_t base = (uint32_t)(row * (2L * NC));
+        const uint32_t base = (uint32_t)((long)(y0 + l) * (2L * NC));
 #pragma unroll
         for (int e = 0; e < E; e++)
-            v[e] = active ? philox_normal_pair(base + 2u * (uint32_t)(t + e * T), (uint32_t)step,
-                                               (uint32_t)(step >> 32), a.stream, a.key0, a.key1)
-                          : make_float2(0.f, 0.f);
-        r2c_store<NC, E, T>(v, active, a.eta + hrow, L, t, tws);
+            v[e] = philox_normal_pair(base + 2u * (uint32_t)(t + e * T), (uint32_t)step, (uint32_t)(step >> 32),
+                                      a.stream, a.key0, a.key1);
+        r2c_tile<NC, E, T>(v, L, t, tws);
+        tile_store<NC, LPB, RS, TH>(a.eta, tile, sy, y0);
     }
 }
 
 // ------------------------------------------------------------ column kernel
 
-template <int N>
+template <int N, int TWD>
 struct FColCfg {
-    static constexpr int E = N >= 512 ? 8 : elems_for(N);
+    static constexpr int E = N >= 4096 ? 16 : (N >= 8 ? 8 : N);
     static constexpr int T = N / E;
+    static constexpr int THREADS = TWD * T;
     static constexpr int LL = lds_len(N);
-    static constexpr bool valid(int twd) { return twd * T >= 64 && twd * T <= 512; }
+    static constexpr bool valid() { return THREADS >= 16 && THREADS <= 512; }
 };
 
 template <int N, int TWD>
-__global__ __launch_bounds__(512) void k_fcols(const FusedInst *__restrict__ insts,
-                                               const FusedBlock *__restrict__ blocks, int parity,
-                                               uint32_t step_rel, int mode, const float2 *__restrict__ sets)
+__global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const FusedInst *__restrict__ insts,
+                                                                   const FusedBlock *__restrict__ blocks,
+                                                                   int parity, uint32_t step_rel, int mode,
+                                                                   const float2 *__restrict__ sets)
 {
-    using C = FColCfg<N>;
+    using C = FColCfg<N, TWD>;
     constexpr int E = C::E, T = C::T;
     __shared__ float2 tws[N];
     __shared__ float2 lds[TWD * C::LL];
@@ -198,11 +239,11 @@ __global__ __launch_bounds__(512) void k_fcols(const FusedInst *__restrict__ ins
     if (blk.inst < 0) return;
     const FusedInst &a = insts[blk.inst];
     const Geom &g = a.g;
-    const int c = threadIdx.x % TWD, t = threadIdx.x / TWD;
+    const int c = threadIdx.x / T, t = threadIdx.x % T;
     const int kx = blk.line + c;
     const bool active = kx < g.nc1;
     float2 *L = lds + c * C::LL;
-    const long stride = g.nc1;
+    const long col = (long)kx * N;
     const bool upd = mode == 1;
     const int ngroups = upd ? a.ngroups : 0;
     const float2 *__restrict__ sin = a.spec[parity];
@@ -210,7 +251,7 @@ __global__ __launch_bounds__(512) void k_fcols(const FusedInst *__restrict__ ins
     float2 cur[E], xv[E], eta[E];
 #pragma unroll
     for (int e = 0; e < E; e++) {
-        const long i = (long)(t + e * T) * stride + kx;
+        const long i = col + t + e * T;
         cur[e] = active ? sin[i] : make_float2(0.f, 0.f);
         xv[e] = (active && ngroups > 0) ? a.X[0][i] : make_float2(0.f, 0.f);
         eta[e] = (active && upd && a.noise_mode == NOISE_REAL) ? a.eta[i] : make_float2(0.f, 0.f);
@@ -226,8 +267,7 @@ __global__ __launch_bounds__(512) void k_fcols(const FusedInst *__restrict__ ins
         for (int gi = 0; gi < ngroups; gi++) {
             if (gi > 0) {
 #pragma unroll
-                for (int e = 0; e < E; e++)
-                    xv[e] = active ? a.X[gi][(long)(t + e * T) * stride + kx] : make_float2(0.f, 0.f);
+                for (int e = 0; e < E; e++) xv[e] = active ? a.X[gi][col + t + e * T] : make_float2(0.f, 0.f);
             }
             fft_line<N, E, -1, N>(xv, L, t, tws);
             const TermRefDev &TR = a.u.terms[a.gterm[gi]];
@@ -266,12 +306,13 @@ __global__ __launch_bounds__(512) void k_fcols(const FusedInst *__restrict__ ins
         if (active) {
             float2 *__restrict__ sout = a.spec[parity ^ 1];
 #pragma unroll
-            for (int e = 0; e < E; e++) sout[(long)(t + e * T) * stride + kx] = cur[e];
+            for (int e = 0; e < E; e++) sout[col + t + e * T] = cur[e];
         }
     }
 #pragma unroll
     for (int f = 0; f < FUSED_MAX_REAL; f++) {
-        if (f < a.nreal) {
+        // workgroups whose columns the mask removes entirely skip the transform
+        if (f < a.nreal && blk.line <= a.kmax[f]) {
             float2 v[E];
             const bool zero = !upd && a.zero[f];
 #pragma unroll
@@ -288,124 +329,245 @@ __global__ __launch_bounds__(512) void k_fcols(const FusedInst *__restrict__ ins
                 v[e] = w;
             }
             fft_line<N, E, +1, N>(v, L, t, tws);
-            if (active) {
+            if (active && kx <= a.kmax[f]) {
 #pragma unroll
-                for (int e = 0; e < E; e++) a.Z[f][(long)(t + e * T) * stride + kx] = v[e];
+                for (int e = 0; e < E; e++) a.Z[f][col + t + e * T] = v[e];
             }
             __syncthreads();
         }
     }
 }
 
+// ------------------------------------------------------------ transposes
+
+// tile (tx, ty) of an instance: columns kx in [tx*S, tx*S+S), rows ky in
+// [ty*S, ty*S+S) of the half spectrum
+__device__ __forceinline__ void tile_origin(const Geom &g, int line, int &x0, int &y0)
+{
+    const int ntx = (g.nc1 + FUSED_TILE - 1) / FUSED_TILE;
+    x0 = (line % ntx) * FUSED_TILE;
+    y0 = (line / ntx) * FUSED_TILE;
+}
+
+__global__ __launch_bounds__(256) void k_ftrans_in(const FusedInst *__restrict__ insts,
+                                                   const FusedBlock *__restrict__ blocks)
+{
+    constexpr int S = FUSED_TILE;
+    __shared__ float2 s[S][S + 1];
+    const FusedBlock blk = blocks[blockIdx.x];
+    if (blk.inst < 0) return;
+    const FusedInst &a = insts[blk.inst];
+    const Geom &g = a.g;
+    int x0, y0;
+    tile_origin(g, blk.line, x0, y0);
+    for (int q = threadIdx.x; q < S * S; q += blockDim.x) {
+        const int i = q % S, j = q / S;
+        if (x0 + i < g.nc1 && y0 + j < g.sy) s[j][i] = a.nat[(long)(y0 + j) * g.nc1 + x0 + i];
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < S * S; q += blockDim.x) {
+        const int j = q % S, i = q / S;
+        if (x0 + i < g.nc1 && y0 + j < g.sy) a.spec[0][(long)(x0 + i) * g.sy + y0 + j] = s[j][i];
+    }
+}
+
+// column-major spec[fin] -> natural spectrum; the derived fields of the
+// last step's starting phi (spec[fin^1]) in natural layout
+__global__ __launch_bounds__(256) void k_fsettle(const FusedInst *__restrict__ insts,
+                                                 const FusedBlock *__restrict__ blocks, int fin)
+{
+    constexpr int S = FUSED_TILE;
+    __shared__ float2 s[S][S + 1];
+    const FusedBlock blk = blocks[blockIdx.x];
+    if (blk.inst < 0) return;
+    const FusedInst &a = insts[blk.inst];
+    const Geom &g = a.g;
+    int x0, y0;
+    tile_origin(g, blk.line, x0, y0);
+    for (int pass = 0; pass < (a.nder > 0 ? 2 : 1); pass++) {
+        const float2 *__restrict__ src = a.spec[pass == 0 ? fin : fin ^ 1];
+        for (int q = threadIdx.x; q < S * S; q += blockDim.x) {
+            const int j = q % S, i = q / S;
+            if (x0 + i < g.nc1 && y0 + j < g.sy) s[j][i] = src[(long)(x0 + i) * g.sy + y0 + j];
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < S * S; q += blockDim.x) {
+            const int i = q % S, j = q / S;
+            const int kx = x0 + i, ky = y0 + j;
+            if (kx < g.nc1 && ky < g.sy) {
+                const long h = (long)ky * g.nc1 + kx;
+                if (pass == 0) {
+                    a.nat[h] = s[j][i];
+                } else {
+                    const Bin b = make_bin(g, kx, ky, 0);
+                    for (int d = 0; d < a.nder; d++) {
+                        const TermRefDev &M = a.dmul[d];
+                        a.dout[d][h] = apply_mult(multiplier(a.dpres, M.pres0, M.npres, M.mult_i, b), M.mult_i,
+                                                  s[j][i]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------ elementwise
 
 // Instances whose step needs no transform (no products, noise drawn in
-// spectral space or absent): one workgroup per spectral row.
-__global__ __launch_bounds__(256) void k_felem(const FusedInst *__restrict__ insts,
-                                               const FusedBlock *__restrict__ blocks, int parity, uint32_t step_rel)
+// spectral space or absent): one thread per bin advances nsteps steps in
+// registers.  The explicit terms are linear in phi or constant, so their sum
+// is (ar + i ai) phi_hat + cst; the step is the dynamic branch of
+// finish_update with D(q) and the noise amplitude evaluated once.
+__global__ __launch_bounds__(FUSED_ELEM_BLOCK) void k_felem(const FusedInst *__restrict__ insts,
+                                                            const FusedBlock *__restrict__ blocks, int parity,
+                                                            uint32_t step_rel, int nsteps)
 {
     const FusedBlock blk = blocks[blockIdx.x];
     if (blk.inst < 0) return;
     const FusedInst &a = insts[blk.inst];
     const Geom &g = a.g;
-    const int ky = blk.line % g.sy, kz = blk.line / g.sy;
-    const uint64_t step = a.step_base + step_rel;
-    const float2 *__restrict__ sin = a.spec[parity];
-    float2 *__restrict__ sout = a.spec[parity ^ 1];
-    for (int kx = threadIdx.x; kx < g.nc1; kx += blockDim.x) {
-        const Bin b = make_bin(g, kx, ky, kz);
-        const float2 cur = sin[b.h];
-        float2 s = make_float2(0.f, 0.f);
-        for (int ti = 0; ti < a.u.nterms; ti++) {
-            const TermRefDev &TR = a.u.terms[ti];
-            const float2 X = TR.kind == TERM_CONST ? (b.h == 0 ? make_float2((float)g.n, 0.f) : make_float2(0.f, 0.f))
-                                                   : cur;
-            s = cadd(s, apply_mult(multiplier(a.u.pres, TR.pres0, TR.npres, TR.mult_i, b), TR.mult_i, X));
-        }
-        float2 eta = make_float2(0.f, 0.f);
-        if (a.noise_mode == NOISE_SPECTRAL)
-            eta = spectral_noise(g, kx, ky, kz, (uint32_t)step, (uint32_t)(step >> 32), a.stream, a.key0, a.key1);
-        sout[b.h] = finish_update(a.u, b, cur, s, eta);
-    }
-}
-
-// Derived constraint fields after a run: psi = P(M_psi phi) from phi's
-// spectrum at the start of the last step (setNotDynamic_k with one
-// single-field term, no implicit operator, no noise).
-__global__ __launch_bounds__(256) void k_fderived(const FusedInst *__restrict__ insts,
-                                                  const FusedBlock *__restrict__ blocks, int parity)
-{
-    const FusedBlock blk = blocks[blockIdx.x];
-    if (blk.inst < 0) return;
-    const FusedInst &a = insts[blk.inst];
-    const Geom &g = a.g;
-    const int ky = blk.line % g.sy, kz = blk.line / g.sy;
-    const float2 *__restrict__ src = a.spec[parity];
-    for (int kx = threadIdx.x; kx < g.nc1; kx += blockDim.x) {
-        const Bin b = make_bin(g, kx, ky, kz);
-        const float2 v = src[b.h];
-        for (int d = 0; d < a.nder; d++) {
-            const TermRefDev &M = a.dmul[d];
-            a.dout[d][b.h] = apply_mult(multiplier(a.dpres, M.pres0, M.npres, M.mult_i, b), M.mult_i, v);
+    const long h = (long)blk.line + threadIdx.x;
+    if (h >= g.nh) return;  // no barriers below
+    const int kx = (int)(h % g.nc1);
+    const long r = h / g.nc1;
+    const int ky = (int)(r % g.sy), kz = (int)(r / g.sy);
+    const Bin b = make_bin(g, kx, ky, kz);
+    const UpdateArgs &u = a.u;
+    float ar = 0.f, ai = 0.f;
+    float2 cst = make_float2(0.f, 0.f);
+    for (int ti = 0; ti < u.nterms; ti++) {
+        const TermRefDev &TR = u.terms[ti];
+        const float m = multiplier(u.pres, TR.pres0, TR.npres, TR.mult_i, b);
+        if (TR.kind == TERM_CONST) {
+            if (b.h == 0) cst = cadd(cst, apply_mult(m, TR.mult_i, make_float2((float)g.n, 0.f)));
+        } else if (TR.mult_i) {
+            ai += m;
+        } else {
+            ar += m;
         }
     }
-}
-
-hipError_t fused_derived_launch(const FusedInst *inst, const FusedBlock *blocks, int nblocks, int parity,
-                                hipStream_t s)
-{
-    hipLaunchKernelGGL(k_fderived, dim3(nblocks), dim3(256), 0, s, inst, blocks, parity);
-    return hipGetLastError();
+    float D = 1.0f;
+    for (int i = 0; i < u.nimp; i++) {
+        const PresDev P = u.imp[i];
+        float th = P.c;
+        if (P.q2n != 0) th *= ipow(b.q2, P.q2n);
+        if (P.invq != 0) th *= ipow(b.iqd, P.invq);
+        D -= u.dt * th;
+    }
+    const bool noisy = u.noisy && a.noise_mode == NOISE_SPECTRAL;
+    const float pn = noisy ? noise_amplitude(u, b) : 0.f;
+    float2 cur = a.spec[parity][h];
+    const uint64_t s0 = a.step_base + step_rel;
+    for (int n = 0; n < nsteps; n++) {
+        if (n == nsteps - 1) {
+            for (int d = 0; d < a.nder; d++) {
+                const TermRefDev &M = a.dmul[d];
+                a.dout[d][h] = apply_mult(multiplier(a.dpres, M.pres0, M.npres, M.mult_i, b), M.mult_i, cur);
+            }
+        }
+        float2 s = cur;
+        s.x += u.dt * (ar * cur.x - ai * cur.y + cst.x);
+        s.y += u.dt * (ar * cur.y + ai * cur.x + cst.y);
+        if (noisy) {
+            const uint64_t step = s0 + (uint64_t)n;
+            const float2 eta =
+                spectral_noise(g, kx, ky, kz, (uint32_t)step, (uint32_t)(step >> 32), a.stream, a.key0, a.key1);
+            s.x += pn * eta.x;
+            s.y += pn * eta.y;
+        }
+        if (u.nimp > 0) {
+            s.x /= D;
+            s.y /= D;
+        }
+        cur = s;
+    }
+    a.spec[parity ^ (nsteps & 1)][h] = cur;
 }
 
 // ------------------------------------------------------------ dispatch
 
 #define FUSED_ROW_CASES(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048)
 #define FUSED_COL_CASES(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
+#define FUSED_LPB(X) X(1) X(2) X(4) X(8) X(16)
+#define FUSED_TWD(X) X(1) X(2) X(4) X(8) X(16)
 
 bool fused_supported(int sx, int sy, int sz) { return sz == 1 && sx >= 16 && sx <= 4096 && sy >= 8 && sy <= 4096; }
 
-int fused_rows_per_block(int nc)
+template <int NC>
+static std::vector<int> row_shapes(int ny)
+{
+    std::vector<int> v;
+#define X(LPB) \
+    if (FRowCfg<NC, LPB>::valid() && LPB <= ny) v.push_back(LPB);
+    FUSED_LPB(X)
+#undef X
+    return v;
+}
+
+std::vector<int> fused_row_shapes(int nc, int ny)
 {
     switch (nc) {
 #define X(NC) \
     case NC:  \
-        return FRowCfg<NC>::LPB;
+        return row_shapes<NC>(ny);
         FUSED_ROW_CASES(X)
 #undef X
     default:
-        return 1;
+        return {};
     }
 }
 
-// tile width: narrowest valid (a single 2D instance has few columns);
-// batches of many instances get the same and rely on their count
-int fused_cols_per_block(int ny)
+template <int N>
+static std::vector<int> col_shapes()
+{
+    std::vector<int> v;
+#define X(TWD) \
+    if (FColCfg<N, TWD>::valid()) v.push_back(TWD);
+    FUSED_TWD(X)
+#undef X
+    return v;
+}
+
+std::vector<int> fused_col_shapes(int ny)
 {
     switch (ny) {
-#define X(N)                             \
-    case N:                              \
-        for (int w : {1, 2, 4, 16, 64})  \
-            if (FColCfg<N>::valid(w)) return w; \
-        return 64;
+#define X(N) \
+    case N:  \
+        return col_shapes<N>();
         FUSED_COL_CASES(X)
 #undef X
     default:
-        return 1;
+        return {};
     }
 }
 
-hipError_t fused_rows_launch(int nc, const FusedInst *inst, const FusedBlock *blocks, int nblocks, uint32_t step_rel,
-                             hipStream_t s)
+template <int NC>
+static hipError_t frows_n(int lpb, const FusedInst *inst, const FusedBlock *blocks, int nblocks, uint32_t step_rel,
+                          hipStream_t s, const float2 *sets)
+{
+#define X(LPB)                                                                                                  \
+    if constexpr (FRowCfg<NC, LPB>::valid()) {                                                                 \
+        if (lpb == LPB) {                                                                                       \
+            hipLaunchKernelGGL((k_frows<NC, LPB>), dim3(nblocks), dim3(FRowCfg<NC, LPB>::THREADS), 0, s, inst, \
+                               blocks, step_rel, sets);                                                         \
+            return hipGetLastError();                                                                           \
+        }                                                                                                       \
+    }
+    FUSED_LPB(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t fused_rows_launch(int nc, int lpb, const FusedInst *inst, const FusedBlock *blocks, int nblocks,
+                             uint32_t step_rel, hipStream_t s)
 {
     const float2 *sets = twiddle_sets(s);
     if (!sets) return hipErrorOutOfMemory;
     switch (nc) {
-#define X(NC)                                                                                               \
-    case NC:                                                                                                \
-        hipLaunchKernelGGL((k_frows<NC>), dim3(nblocks), dim3(FRowCfg<NC>::THREADS), 0, s, inst, blocks,   \
-                           step_rel, sets);                                                                 \
-        return hipGetLastError();
+#define X(NC) \
+    case NC:  \
+        return frows_n<NC>(lpb, inst, blocks, nblocks, step_rel, s, sets);
         FUSED_ROW_CASES(X)
 #undef X
     default:
@@ -414,32 +576,31 @@ hipError_t fused_rows_launch(int nc, const FusedInst *inst, const FusedBlock *bl
 }
 
 template <int N>
-static hipError_t fcols_n(const FusedInst *inst, const FusedBlock *blocks, int nblocks, int parity,
+static hipError_t fcols_n(int twd, const FusedInst *inst, const FusedBlock *blocks, int nblocks, int parity,
                           uint32_t step_rel, int mode, hipStream_t s, const float2 *sets)
 {
-    const int w = fused_cols_per_block(N);
-#define L(W)                                                                                                 \
-    if constexpr (FColCfg<N>::valid(W)) {                                                                    \
-        if (w == W) {                                                                                        \
-            hipLaunchKernelGGL((k_fcols<N, W>), dim3(nblocks), dim3(W * FColCfg<N>::T), 0, s, inst, blocks,  \
-                               parity, step_rel, mode, sets);                                                \
-            return hipGetLastError();                                                                        \
-        }                                                                                                    \
+#define X(TWD)                                                                                                \
+    if constexpr (FColCfg<N, TWD>::valid()) {                                                                \
+        if (twd == TWD) {                                                                                     \
+            hipLaunchKernelGGL((k_fcols<N, TWD>), dim3(nblocks), dim3(FColCfg<N, TWD>::THREADS), 0, s, inst, \
+                               blocks, parity, step_rel, mode, sets);                                         \
+            return hipGetLastError();                                                                         \
+        }                                                                                                     \
     }
-    L(1) L(2) L(4) L(16) L(64)
-#undef L
+    FUSED_TWD(X)
+#undef X
     return hipErrorInvalidValue;
 }
 
-hipError_t fused_cols_launch(int ny, const FusedInst *inst, const FusedBlock *blocks, int nblocks, int parity,
-                             uint32_t step_rel, int mode, hipStream_t s)
+hipError_t fused_cols_launch(int ny, int twd, const FusedInst *inst, const FusedBlock *blocks, int nblocks,
+                             int parity, uint32_t step_rel, int mode, hipStream_t s)
 {
     const float2 *sets = twiddle_sets(s);
     if (!sets) return hipErrorOutOfMemory;
     switch (ny) {
 #define X(N) \
     case N:  \
-        return fcols_n<N>(inst, blocks, nblocks, parity, step_rel, mode, s, sets);
+        return fcols_n<N>(twd, inst, blocks, nblocks, parity, step_rel, mode, s, sets);
         FUSED_COL_CASES(X)
 #undef X
     default:
@@ -447,10 +608,23 @@ hipError_t fused_cols_launch(int ny, const FusedInst *inst, const FusedBlock *bl
     }
 }
 
-hipError_t fused_elem_launch(const FusedInst *inst, const FusedBlock *blocks, int nblocks, int parity,
-                             uint32_t step_rel, hipStream_t s)
+hipError_t fused_transpose_in_launch(const FusedInst *inst, const FusedBlock *blocks, int nblocks, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_felem, dim3(nblocks), dim3(256), 0, s, inst, blocks, parity, step_rel);
+    hipLaunchKernelGGL(k_ftrans_in, dim3(nblocks), dim3(256), 0, s, inst, blocks);
+    return hipGetLastError();
+}
+
+hipError_t fused_settle_launch(const FusedInst *inst, const FusedBlock *blocks, int nblocks, int fin, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fsettle, dim3(nblocks), dim3(256), 0, s, inst, blocks, fin);
+    return hipGetLastError();
+}
+
+hipError_t fused_elem_launch(const FusedInst *inst, const FusedBlock *blocks, int nblocks, int parity,
+                             uint32_t step_rel, int nsteps, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_felem, dim3(nblocks), dim3(FUSED_ELEM_BLOCK), 0, s, inst, blocks, parity, step_rel,
+                       nsteps);
     return hipGetLastError();
 }
 

@@ -116,6 +116,7 @@ extern "C" int cupss_system_destroy(cupss_system s)
     for (auto &T : s->t) hipFree(T.xhat);
     for (auto *p : s->fz) hipFree(p);
     for (auto *p : s->fx) hipFree(p);
+    for (auto *p : s->ft) hipFree(p);
     hipFree(s->scratch);
     hipFree(s->rtmp);
     hipFree(s->full);

@@ -75,6 +75,7 @@ struct cupss_system_t {
     cupss::FusedPlan plan;
     float2 *fz[cupss::FUSED_MAX_REAL] = {nullptr, nullptr, nullptr};
     float2 *fx[cupss::FUSED_MAX_GROUPS] = {nullptr, nullptr, nullptr, nullptr};
+    float2 *ft[2] = {nullptr, nullptr};  // phi's spectrum, column-major, during fused runs
     long fused_steps = 0;     // steps taken by the fused path (diagnostics)
     uint64_t step = 0;
     int device = 0;

@@ -163,6 +163,13 @@ int cupss_batch_add(cupss_batch b, cupss_system s);
 int cupss_batch_size(cupss_batch b);
 /* 0 = direct launches; K (even, >= 2) = capture K steps in a hipGraph */
 int cupss_batch_set_graph_steps(cupss_batch b, int k);
+/* 1 (default) = time the tile shapes and chunk sizes on the batch's own
+ * data at its first run and keep the fastest (results are identical for
+ * every choice); 0 = fixed default shapes, one chunk per size class */
+int cupss_batch_set_autotune(cupss_batch b, int on);
+/* text description of the schedule in use (size classes, tile shapes,
+ * chunks) into buf (len bytes, NUL-terminated) */
+int cupss_batch_schedule(cupss_batch b, char *buf, int len);
 int cupss_batch_advance(cupss_batch b, int nsteps);
 int cupss_batch_time_advance(cupss_batch b, int nsteps, float *ms);
 int cupss_batch_synchronize(cupss_batch b);
