@@ -53,7 +53,7 @@ def diffusion(S, P, dims, x0, dt=0.01, **kw):
 
 
 def ensemble_members(count=512, seed=0):
-    """Config 4: 512 independent 2D instances, side uniform in {64,128,256,512},
+    """The ensemble: 512 independent 2D instances, side uniform in {64,128,256,512},
     Edwards-Wilkinson for even index (lambda = 0) and KPZ for odd index
     (lambda ~ U[0, 2]), nu ~ U[0.5, 2], D = 0.01, dt = 0.005, h0 = 0."""
     rng = np.random.default_rng(seed)
@@ -96,6 +96,7 @@ WORKLOADS = {
     "diffusion": dict(build=diffusion, dims=(256, 256, 1), name="2D diffusion 256^2 (reference example 01)",
                       cpu_steps=2000),
     "ensemble": dict(name="batched EW/KPZ ensemble, 512 instances of 64^2..512^2", cpu_steps=0),
+    "serve": dict(name="served model-H stream, Poisson arrivals, sides 128..1024", cpu_steps=0),
 }
 
 
@@ -135,6 +136,92 @@ def bench_ensemble(args, C):
     return value, ms, npts, {"plain_value": plain_value, "speedup_vs_plain": value / plain_value,
                              "instances": len(members), "points": npts, "noise": args.noise,
                              "graph_steps": args.graph}
+
+
+SERVE_SIZES = (128, 256, 512, 1024)
+SERVE_WEIGHTS = (0.4, 0.3, 0.2, 0.1)
+SERVE_STEPS = (1e3, 1e5)
+
+
+def serve_jobs(count, seed=0):
+    """The served stream's jobs: model H, side in {128,256,512,1024} with
+    weights 0.4/0.3/0.2/0.1, steps log-uniform in [1e3, 1e5], phi0 ~ U(+-0.1)
+    from a per-job seed."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice(SERVE_SIZES, size=count, p=SERVE_WEIGHTS)
+    steps = np.exp(rng.uniform(np.log(SERVE_STEPS[0]), np.log(SERVE_STEPS[1]), size=count))
+    return [(int(n), int(k), 1000 + i) for i, (n, k) in enumerate(zip(sizes, steps))]
+
+
+def bench_serve(args, C):
+    """Poisson arrivals of model-H jobs at 80% of the measured capacity,
+    served with continuous batching under a device-memory budget."""
+    from cupss_amd.systems import build, spec_model_h
+    est = {}
+    for n in SERVE_SIZES:
+        s, _ = build(spec_model_h(n, seed=1))
+        est[n] = s.device_bytes()
+        s.close()
+    budget = int(args.budget_gib * (1 << 30))
+
+    def submit(srv, jobs, arrivals, sink=None):
+        for (n, k, seed), t in zip(jobs, arrivals):
+            srv.submit(lambda n=n, seed=seed: build(spec_model_h(n, seed=seed))[0], k, arrival=t,
+                       bytes_estimate=est[n], done=sink)
+
+    # capacity: a saturated mix (two jobs of each size, all present at t = 0)
+    cal = [(n, 400, 7 + i) for i, n in enumerate(SERVE_SIZES * 2)]
+    srv = C.Server(budget, quantum=args.quantum, max_active=args.max_active)
+    submit(srv, cal, [0.0] * len(cal))
+    srv.run()
+    st = srv.stats()
+    srv.close()
+    capacity = st["point_steps"] / st["wall_s"]
+    mean_points = sum(w * n * n for n, w in zip(SERVE_SIZES, SERVE_WEIGHTS))
+    a, b = SERVE_STEPS
+    mean_steps = (b - a) / np.log(b / a)
+    rate = args.load * capacity / (mean_points * mean_steps)
+    jobs = serve_jobs(args.jobs)
+    arrivals = np.cumsum(np.random.default_rng(1).exponential(1.0 / rate, len(jobs)))
+    print("bench: serve %d model-H jobs, capacity %.3g point-steps/s, rate %.3g jobs/s (load %.2f), "
+          "budget %.1f GiB" % (len(jobs), capacity, rate, args.load, args.budget_gib), file=sys.stderr, flush=True)
+    srv = C.Server(budget, quantum=args.quantum, max_active=args.max_active)
+    submit(srv, jobs, arrivals)
+    srv.run()
+    st = srv.stats()
+    tts = np.array([srv.times(j)[2] - srv.times(j)[0] for j in range(len(jobs))])
+    wait = np.array([srv.times(j)[1] - srv.times(j)[0] for j in range(len(jobs))])
+    srv.close()
+    value = st["point_steps"] / st["wall_s"]
+    extra = {"jobs": len(jobs), "arrival_rate_per_s": rate, "load": args.load, "capacity": capacity,
+             "p50_tts_s": float(np.percentile(tts, 50)), "p99_tts_s": float(np.percentile(tts, 99)),
+             "p50_queue_s": float(np.percentile(wait, 50)), "p99_queue_s": float(np.percentile(wait, 99)),
+             "wall_s": st["wall_s"], "budget_bytes": budget, "peak_bytes": st["peak_bytes"],
+             "peak_active": st["peak_active"], "quantum": args.quantum}
+    return value, st["wall_s"] * 1e3, extra, jobs
+
+
+def cpu_baseline_serve(jobs):
+    """The oracle (FP32 restatement of the reference CPU scheme, 1 core):
+    one model-H step per size, extrapolated to the served jobs (the
+    reference runs one job at a time)."""
+    from cupss_amd.systems import spec_model_h
+    from oracle import oracle as O
+    from cupss_amd.systems import build
+    per = {}
+    for n in SERVE_SIZES:
+        spec = dict(spec_model_h(n, seed=1))
+        spec["precision"] = "f32"
+        s, _ = build(spec, O.System, O.pres)
+        s.step(1)
+        t0 = time.perf_counter()
+        s.step(1)
+        per[n] = time.perf_counter() - t0
+    total_t = sum(per[n] * k for n, k, _ in jobs)
+    pts = sum(n * n * k for n, k, _ in jobs)
+    return {"value": pts / total_t, "unit": "grid-point-steps/s", "cores": 1, "kind": "port",
+            "sample": "one model-H step per size (128..1024) after 1 warm-up step, oracle FP32 (C2C, radix-2), "
+                      "extrapolated to the served jobs' sizes and step counts, %s" % _cpu_name()}
 
 
 def cpu_baseline_ensemble(members, budget_s=20.0):
@@ -203,6 +290,11 @@ def main():
     ap.add_argument("--noise", default="spectral", choices=["real", "spectral"],
                     help="ensemble noise draw: real-space Philox + R2C, or spectral-space Philox")
     ap.add_argument("--graph", type=int, default=20, help="ensemble: steps per hipGraph (0 = direct launches)")
+    ap.add_argument("--jobs", type=int, default=32, help="serve: number of jobs")
+    ap.add_argument("--load", type=float, default=0.8, help="serve: arrival rate / measured capacity")
+    ap.add_argument("--budget-gib", type=float, default=4.0, help="serve: device-memory budget")
+    ap.add_argument("--quantum", type=int, default=100, help="serve: steps per scheduling round")
+    ap.add_argument("--max-active", type=int, default=64, help="serve: resident jobs at most")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build runs on a single GPU (--gpus 1); "
@@ -211,6 +303,22 @@ def main():
 
     import cupss_amd as C
     wl = WORKLOADS[args.workload]
+    if args.workload == "serve":
+        value, ms, extra, jobs = bench_serve(args, C)
+        achieved = BYTES_PER_POINT_STEP * value / 1e9
+        out = {"metric": "grid-point-steps/s", "value": value, "unit": "grid-point-steps/s", "n_gpus": 1,
+               "steps": extra["jobs"], "warmup": 0, "ms_per_step": None,
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+               "data": "synthetic", "config": dict({"workload": wl["name"], "dt": 0.01,
+                                                     "path": "server: fused batch + general program graphs"},
+                                                    **extra),
+               "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                            "per": "aggregate over the served run, 8 B/point-step algorithmic"}}
+        if not args.no_cpu:
+            out["cpu_baseline"] = cpu_baseline_serve(jobs)
+        print(json.dumps(out), flush=True)
+        return
     if args.workload == "ensemble":
         value, ms, npts, extra = bench_ensemble(args, C)
         achieved = BYTES_PER_POINT_STEP * value / 1e9

@@ -78,6 +78,17 @@ _SIGS = {
     "cupss_batch_advance": (_I, [_VP, _I]),
     "cupss_batch_time_advance": (_I, [_VP, _I, _FP]),
     "cupss_batch_synchronize": (_I, [_VP]),
+    "cupss_batch_remove": (_I, [_VP, _VP]),
+    "cupss_system_set_graph_steps": (_I, [_VP, _I]),
+    "cupss_system_device_bytes": (_I, [_VP, ctypes.POINTER(ctypes.c_size_t)]),
+    "cupss_server_create": (_I, [ctypes.c_size_t, _I, _I, ctypes.POINTER(_VP)]),
+    "cupss_server_destroy": (_I, [_VP]),
+    "cupss_server_submit": (_I, [_VP, ctypes.c_double, _I, ctypes.c_size_t, _VP, _VP, _VP, ctypes.POINTER(_I)]),
+    "cupss_server_run": (_I, [_VP]),
+    "cupss_server_job_times": (_I, [_VP, _I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_double)]),
+    "cupss_server_stats": (_I, [_VP, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_I)]),
     "cupss_field_spectrum_device": (_I, [_VP, _I, ctypes.POINTER(_VP)]),
     "cupss_r2c": (_I, [_I, _I, _I, _VP, _VP, _VP]),
     "cupss_c2r": (_I, [_I, _I, _I, _VP, _I, _F, _VP, _VP, _VP]),
@@ -163,16 +174,44 @@ class System:
         self.n = sx * sy * sz
         self.nh = (sx // 2 + 1) * sy * sz
 
+    owned = True
+
+    @classmethod
+    def borrow(cls, handle, dims):
+        """A view of a system owned elsewhere (e.g. by a server); never destroys it."""
+        s = cls.__new__(cls)
+        s.L = lib()
+        s.h = ctypes.c_void_p(handle)
+        s.owned = False
+        s.dims = tuple(dims)
+        s.n = dims[0] * dims[1] * dims[2]
+        s.nh = (dims[0] // 2 + 1) * dims[1] * dims[2]
+        return s
+
+    def detach(self):
+        """Hand the system over (e.g. to a server, which destroys it); returns the raw handle."""
+        h = self.h.value
+        self.h = ctypes.c_void_p()
+        return h
+
     def close(self):
-        if getattr(self, "h", None) and self.h.value:
+        if getattr(self, "h", None) and self.h.value and self.owned:
             self.L.cupss_system_destroy(self.h)
-            self.h = ctypes.c_void_p()
+        self.h = ctypes.c_void_p()
 
     def __del__(self):
         try:
             self.close()
         except Exception:
             pass
+
+    def set_graph_steps(self, k):
+        check(self.L.cupss_system_set_graph_steps(self.h, k))
+
+    def device_bytes(self):
+        b = ctypes.c_size_t()
+        check(self.L.cupss_system_device_bytes(self.h, ctypes.byref(b)))
+        return b.value
 
     def add_field(self, dynamic):
         fid = ctypes.c_int()
@@ -293,6 +332,81 @@ class Batch:
     def close(self):
         if getattr(self, "h", None) and self.h.value:
             self.L.cupss_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_BUILD = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p))
+_DONE = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p)
+
+
+class Server:
+    """Serving loop with continuous batching (cupss_server_* of the C-ABI).
+
+    submit(build, nsteps, arrival, bytes_estimate, done): `build()` returns a
+    prepared System (built only when the job is admitted); `done(system)`
+    gets a borrowed view of the finished system before it is destroyed.
+    """
+
+    def __init__(self, budget_bytes, quantum=100, max_active=64):
+        self.L = lib()
+        self.h = ctypes.c_void_p()
+        check(self.L.cupss_server_create(int(budget_bytes), quantum, max_active, ctypes.byref(self.h)))
+        self.jobs = []
+        self.errors = []
+        self._build_cb = _BUILD(self._on_build)
+        self._done_cb = _DONE(self._on_done)
+
+    def _on_build(self, user, job, out):
+        try:
+            s = self.jobs[job]["build"]()
+            self.jobs[job]["dims"] = s.dims
+            out[0] = s.detach()
+            return 0
+        except Exception as e:  # reported by run()
+            self.errors.append(e)
+            return -1
+
+    def _on_done(self, user, job, handle):
+        try:
+            cb = self.jobs[job]["done"]
+            if cb is not None:
+                cb(System.borrow(handle, self.jobs[job]["dims"]))
+        except Exception as e:
+            self.errors.append(e)
+
+    def submit(self, build, nsteps, arrival=0.0, bytes_estimate=0, done=None):
+        jid = ctypes.c_int()
+        self.jobs.append({"build": build, "done": done, "nsteps": nsteps})
+        check(self.L.cupss_server_submit(self.h, float(arrival), int(nsteps), int(bytes_estimate),
+                                         ctypes.cast(self._build_cb, _VP), ctypes.cast(self._done_cb, _VP),
+                                         None, ctypes.byref(jid)))
+        return jid.value
+
+    def run(self):
+        rc = self.L.cupss_server_run(self.h)
+        if self.errors:
+            raise self.errors[0]
+        check(rc)
+
+    def times(self, job):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        check(self.L.cupss_server_job_times(self.h, job, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def stats(self):
+        w, p, b, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_size_t(), ctypes.c_int()
+        check(self.L.cupss_server_stats(self.h, ctypes.byref(w), ctypes.byref(p), ctypes.byref(b), ctypes.byref(n)))
+        return {"wall_s": w.value, "point_steps": p.value, "peak_bytes": b.value, "peak_active": n.value}
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.L.cupss_server_destroy(self.h)
             self.h = ctypes.c_void_p()
 
     def __del__(self):

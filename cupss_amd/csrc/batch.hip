@@ -68,7 +68,7 @@ struct FusedBatch {
     std::vector<Class> classes;
     std::vector<int> elem;      // elementwise instances
     std::vector<char> is_elem;  // per instance
-    FusedBlock *eb = nullptr;   // elementwise: FUSED_ELEM_BLOCK bins per block
+    FusedBlock *eb = nullptr;   // elementwise: FUSED_ELEM_BLOCK pairs of bins per block
     int neb = 0;
     FusedBlock *tb = nullptr;   // transpose tiles of the transform instances
     int ntb = 0;
@@ -363,7 +363,7 @@ static int build(FusedBatch &B)
         if (is_elementwise(a)) {
             B.elem.push_back((int)i);
             B.is_elem[i] = 1;
-            for (long h0 = 0; h0 < a.g.nh; h0 += FUSED_ELEM_BLOCK) er.push_back(FusedBlock{(int)i, (int)h0});
+            for (long h0 = 0; h0 < a.g.nh / 2; h0 += FUSED_ELEM_BLOCK) er.push_back(FusedBlock{(int)i, (int)h0});
             continue;
         }
         auto key = std::make_pair(a.g.nc, a.g.sy);
@@ -641,6 +641,10 @@ int fused_steps(cupss_system s, int nsteps)
         B->stream = s->stream;
         B->built = false;
     }
+    if (B->graph_steps != s->graph_steps) {
+        B->graph_steps = s->graph_steps;
+        B->drop_graph();
+    }
     return run(*B, nsteps);
 }
 
@@ -709,6 +713,18 @@ extern "C" int cupss_batch_add(cupss_batch b, cupss_system s)
         HIP_TRY(hipStreamSynchronize(s->stream));
     }
     b->b.sys.push_back(s);
+    b->b.built = false;
+    return CUPSS_OK;
+}
+
+extern "C" int cupss_batch_remove(cupss_batch b, cupss_system s)
+{
+    if (!b || !s) return fail(CUPSS_ERR_INVALID, "null batch or system");
+    auto &v = b->b.sys;
+    auto it = std::find(v.begin(), v.end(), s);
+    if (it == v.end()) return fail(CUPSS_ERR_INVALID, "system is not in the batch");
+    HIP_TRY(hipStreamSynchronize(b->b.stream));
+    v.erase(it);
     b->b.built = false;
     return CUPSS_OK;
 }

@@ -143,6 +143,7 @@ __global__ __launch_bounds__((FRowCfg<NC, LPB>::THREADS)) void k_frows(const Fus
 {
     using C = FRowCfg<NC, LPB>;
     constexpr int E = C::E, T = C::T, RS = C::RS, TH = C::THREADS;
+    static_assert(T % 2 == 0 && E % 2 == 0, "noise lanes are paired");
     __shared__ float2 tws[2 * NC];
     __shared__ float2 tile[LPB * RS];
     const FusedBlock blk = blocks[blockIdx.x];
@@ -202,13 +203,26 @@ __global__ __launch_bounds__((FRowCfg<NC, LPB>::THREADS)) void k_frows(const Fus
         __syncthreads();
     }
     if (a.noise_mode == NOISE_REAL) {
+        // Cells 2k, 2k+1 of row y0+l, k = t + e*T.  The Philox call with
+        // counter (row*NC + k) >> 1 serves k and k^1, which sits in lane
+        // t^1 (T is even): the even lane computes the calls of elements
+        // e < E/2, the odd lane those of e >= E/2, and each hands its
+        // partner the Box-Muller pair that the partner's cells take.
         const uint64_t step = a.step_base + step_rel;
+        const uint32_t rowbase = (uint32_t)((long)(y0 + l) * NC);
+        const bool even = (t & 1) == 0;
         float2 v[E];
-        const uint32_t base = (uint32_t)((long)(y0 + l) * (2L * NC));
 #pragma unroll
-        for (int e = 0; e < E; e++)
-            v[e] = philox_normal_pair(base + 2u * (uint32_t)(t + e * T), (uint32_t)step, (uint32_t)(step >> 32),
-                                      a.stream, a.key0, a.key1);
+        for (int e2 = 0; e2 < E / 2; e2++) {
+            const uint32_t k = (uint32_t)((t & ~1) + (even ? e2 : e2 + E / 2) * T);
+            const Philox4 o = philox4x32_10((rowbase + k) >> 1, (uint32_t)step, (uint32_t)(step >> 32), a.stream,
+                                            a.key0, a.key1);
+            const float2 ze = box_muller(o.x[0], o.x[1]), zo = box_muller(o.x[2], o.x[3]);
+            const float2 own = even ? ze : zo, snd = even ? zo : ze;
+            const float2 rcv = make_float2(__shfl_xor(snd.x, 1), __shfl_xor(snd.y, 1));
+            v[e2] = even ? own : rcv;
+            v[e2 + E / 2] = even ? rcv : own;
+        }
         r2c_tile<NC, E, T>(v, L, t, tws);
         tile_store<NC, LPB, RS, TH>(a.eta, tile, sy, y0);
     }
@@ -283,10 +297,11 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
             fft_line<N, E, -1, N>(eta, L, t, tws);
             __syncthreads();
         } else if (a.noise_mode == NOISE_SPECTRAL) {
+            // ky and ky + N/2 are elements e and e + E/2 of this lane
 #pragma unroll
-            for (int e = 0; e < E; e++)
-                eta[e] = spectral_noise(g, kx, t + e * T, 0, (uint32_t)step, (uint32_t)(step >> 32), a.stream,
-                                        a.key0, a.key1);
+            for (int e = 0; e < E / 2; e++)
+                spectral_noise_pair(g, kx, t + e * T, 0, (uint32_t)step, (uint32_t)(step >> 32), a.stream, a.key0,
+                                    a.key1, eta[e], eta[e + E / 2]);
         }
 #pragma unroll
         for (int e = 0; e < E; e++) {
@@ -415,10 +430,12 @@ __global__ __launch_bounds__(256) void k_fsettle(const FusedInst *__restrict__ i
 // ------------------------------------------------------------ elementwise
 
 // Instances whose step needs no transform (no products, noise drawn in
-// spectral space or absent): one thread per bin advances nsteps steps in
-// registers.  The explicit terms are linear in phi or constant, so their sum
-// is (ar + i ai) phi_hat + cst; the step is the dynamic branch of
-// finish_update with D(q) and the noise amplitude evaluated once.
+// spectral space or absent): one thread per pair of bins (kx, j, kz),
+// (kx, j + sy/2, kz) -- the pair that shares a noise draw -- advances
+// nsteps steps in registers.  The explicit terms are linear in phi or
+// constant, so their sum is (ar + i ai) phi_hat + cst; the step is the
+// dynamic branch of finish_update with D(q) and the noise amplitude
+// evaluated once.
 __global__ __launch_bounds__(FUSED_ELEM_BLOCK) void k_felem(const FusedInst *__restrict__ insts,
                                                             const FusedBlock *__restrict__ blocks, int parity,
                                                             uint32_t step_rel, int nsteps)
@@ -427,62 +444,81 @@ __global__ __launch_bounds__(FUSED_ELEM_BLOCK) void k_felem(const FusedInst *__r
     if (blk.inst < 0) return;
     const FusedInst &a = insts[blk.inst];
     const Geom &g = a.g;
-    const long h = (long)blk.line + threadIdx.x;
-    if (h >= g.nh) return;  // no barriers below
-    const int kx = (int)(h % g.nc1);
-    const long r = h / g.nc1;
-    const int ky = (int)(r % g.sy), kz = (int)(r / g.sy);
-    const Bin b = make_bin(g, kx, ky, kz);
+    const int hy = g.sy / 2;  // sy >= 8 for fused instances
+    const long pidx = (long)blk.line + threadIdx.x;
+    if (pidx >= g.nh / 2) return;  // no barriers below
+    const int kx = (int)(pidx % g.nc1);
+    const long r = pidx / g.nc1;
+    const int j = (int)(r % hy), kz = (int)(r / hy);
     const UpdateArgs &u = a.u;
-    float ar = 0.f, ai = 0.f;
-    float2 cst = make_float2(0.f, 0.f);
-    for (int ti = 0; ti < u.nterms; ti++) {
-        const TermRefDev &TR = u.terms[ti];
-        const float m = multiplier(u.pres, TR.pres0, TR.npres, TR.mult_i, b);
-        if (TR.kind == TERM_CONST) {
-            if (b.h == 0) cst = cadd(cst, apply_mult(m, TR.mult_i, make_float2((float)g.n, 0.f)));
-        } else if (TR.mult_i) {
-            ai += m;
-        } else {
-            ar += m;
-        }
-    }
-    float D = 1.0f;
-    for (int i = 0; i < u.nimp; i++) {
-        const PresDev P = u.imp[i];
-        float th = P.c;
-        if (P.q2n != 0) th *= ipow(b.q2, P.q2n);
-        if (P.invq != 0) th *= ipow(b.iqd, P.invq);
-        D -= u.dt * th;
-    }
     const bool noisy = u.noisy && a.noise_mode == NOISE_SPECTRAL;
-    const float pn = noisy ? noise_amplitude(u, b) : 0.f;
-    float2 cur = a.spec[parity][h];
+    Bin b[2];
+    float ar[2], ai[2], D[2], pn[2];
+    float2 cst[2], cur[2];
+#pragma unroll
+    for (int w = 0; w < 2; w++) {
+        b[w] = make_bin(g, kx, j + w * hy, kz);
+        ar[w] = 0.f;
+        ai[w] = 0.f;
+        cst[w] = make_float2(0.f, 0.f);
+        for (int ti = 0; ti < u.nterms; ti++) {
+            const TermRefDev &TR = u.terms[ti];
+            const float m = multiplier(u.pres, TR.pres0, TR.npres, TR.mult_i, b[w]);
+            if (TR.kind == TERM_CONST) {
+                if (b[w].h == 0) cst[w] = cadd(cst[w], apply_mult(m, TR.mult_i, make_float2((float)g.n, 0.f)));
+            } else if (TR.mult_i) {
+                ai[w] += m;
+            } else {
+                ar[w] += m;
+            }
+        }
+        D[w] = 1.0f;
+        for (int i = 0; i < u.nimp; i++) {
+            const PresDev P = u.imp[i];
+            float th = P.c;
+            if (P.q2n != 0) th *= ipow(b[w].q2, P.q2n);
+            if (P.invq != 0) th *= ipow(b[w].iqd, P.invq);
+            D[w] -= u.dt * th;
+        }
+        pn[w] = noisy ? noise_amplitude(u, b[w]) : 0.f;
+        cur[w] = a.spec[parity][b[w].h];
+    }
     const uint64_t s0 = a.step_base + step_rel;
     for (int n = 0; n < nsteps; n++) {
         if (n == nsteps - 1) {
             for (int d = 0; d < a.nder; d++) {
                 const TermRefDev &M = a.dmul[d];
-                a.dout[d][h] = apply_mult(multiplier(a.dpres, M.pres0, M.npres, M.mult_i, b), M.mult_i, cur);
+#pragma unroll
+                for (int w = 0; w < 2; w++)
+                    a.dout[d][b[w].h] =
+                        apply_mult(multiplier(a.dpres, M.pres0, M.npres, M.mult_i, b[w]), M.mult_i, cur[w]);
             }
         }
-        float2 s = cur;
-        s.x += u.dt * (ar * cur.x - ai * cur.y + cst.x);
-        s.y += u.dt * (ar * cur.y + ai * cur.x + cst.y);
+        float2 eta[2] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
         if (noisy) {
             const uint64_t step = s0 + (uint64_t)n;
-            const float2 eta =
-                spectral_noise(g, kx, ky, kz, (uint32_t)step, (uint32_t)(step >> 32), a.stream, a.key0, a.key1);
-            s.x += pn * eta.x;
-            s.y += pn * eta.y;
+            spectral_noise_pair(g, kx, j, kz, (uint32_t)step, (uint32_t)(step >> 32), a.stream, a.key0, a.key1,
+                                eta[0], eta[1]);
         }
-        if (u.nimp > 0) {
-            s.x /= D;
-            s.y /= D;
+#pragma unroll
+        for (int w = 0; w < 2; w++) {
+            const float2 c = cur[w];
+            float2 s = c;
+            s.x += u.dt * (ar[w] * c.x - ai[w] * c.y + cst[w].x);
+            s.y += u.dt * (ar[w] * c.y + ai[w] * c.x + cst[w].y);
+            if (noisy) {
+                s.x += pn[w] * eta[w].x;
+                s.y += pn[w] * eta[w].y;
+            }
+            if (u.nimp > 0) {
+                s.x /= D[w];
+                s.y /= D[w];
+            }
+            cur[w] = s;
         }
-        cur = s;
     }
-    a.spec[parity ^ (nsteps & 1)][h] = cur;
+#pragma unroll
+    for (int w = 0; w < 2; w++) a.spec[parity ^ (nsteps & 1)][b[w].h] = cur[w];
 }
 
 // ------------------------------------------------------------ dispatch

@@ -18,7 +18,7 @@ constexpr int FUSED_MAX_FAC = 4;     // factors of one product
 constexpr int FUSED_MAX_LPRES = 16;  // prefactors of the derived-field multipliers
 constexpr int FUSED_MAX_DERIVED = 4; // derived fields written back after a run
 constexpr int FUSED_TILE = 32;       // side of the transpose tiles
-constexpr int FUSED_ELEM_BLOCK = 256;  // bins per workgroup of the elementwise kernel
+constexpr int FUSED_ELEM_BLOCK = 256;  // bin pairs per workgroup of the elementwise kernel
 
 enum NoiseMode : int {
     NOISE_NONE = 0,
@@ -90,7 +90,7 @@ hipError_t fused_transpose_in_launch(const FusedInst *inst, const FusedBlock *bl
 hipError_t fused_settle_launch(const FusedInst *inst, const FusedBlock *blocks, int nblocks, int fin,
                                hipStream_t s);
 // nsteps steps of instances with no products and no real-space noise
-// (blocks: FUSED_ELEM_BLOCK bins each); derived fields from the last step's start
+// (blocks: FUSED_ELEM_BLOCK bin pairs each); derived fields from the last step's start
 hipError_t fused_elem_launch(const FusedInst *inst, const FusedBlock *blocks, int nblocks, int parity,
                              uint32_t step_rel, int nsteps, hipStream_t s);
 

@@ -107,6 +107,8 @@ extern "C" int cupss_system_destroy(cupss_system s)
     if (!s) return CUPSS_OK;
     if (s->stream) hipStreamSynchronize(s->stream);
     fused_release(s);
+    if (s->gexec) hipGraphExecDestroy(s->gexec);
+    if (s->graph) hipGraphDestroy(s->graph);
     for (auto &F : s->f) {
         hipFree(F.spec[0]);
         hipFree(F.spec[1]);
@@ -593,6 +595,72 @@ static bool hooks_present(cupss_system s)
     return false;
 }
 
+static void drop_general_graph(cupss_system s)
+{
+    if (s->gexec) hipGraphExecDestroy(s->gexec);
+    if (s->graph) hipGraphDestroy(s->graph);
+    s->gexec = nullptr;
+    s->graph = nullptr;
+}
+
+// The general step program, replayed from a hipGraph of K steps where that
+// is valid: the launches of a step depend on host state only through the
+// buffer parities (each field's spectrum alternates between two buffers;
+// after K even steps they are back where they started) and, for noise,
+// through the step counter -- so noisy systems and systems with hooks run
+// direct launches, and a replay starts only at the parities of the capture
+// (one direct step realigns them).  Capturing records K steps without
+// running them (the host state already moves on by K steps); launching the
+// graph once then makes the device catch up.
+static bool graph_parity(cupss_system s)
+{
+    if (s->graph_cur.size() != s->f.size()) return false;
+    for (size_t i = 0; i < s->f.size(); i++)
+        if (s->f[i].cur != s->graph_cur[i]) return false;
+    return true;
+}
+
+static int general_advance(cupss_system s, int nsteps)
+{
+    const int K = s->graph_steps;
+    bool noisy = false;
+    for (auto &F : s->f) noisy = noisy || F.noisy;
+    if (K < 2 || nsteps < K || noisy || hooks_present(s)) return general_steps(s, nsteps);
+    for (int i = 0; i < (int)s->f.size(); i++)
+        if (int r = materialize(s, i)) return r;
+    if (s->gexec && (s->graph_gen != s->gen || s->graph_stream != s->stream)) drop_general_graph(s);
+    int done = 0;
+    if (s->gexec && !graph_parity(s)) {
+        if (int r = general_steps(s, 1)) return r;
+        done = 1;
+    }
+    if (!s->gexec) {
+        s->graph_cur.clear();
+        for (auto &F : s->f) s->graph_cur.push_back(F.cur);
+        if (!twiddles(s->stream)) return fail(CUPSS_ERR_NOMEM, "twiddle tables");
+        HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = general_steps(s, K);
+        hipGraph_t g = nullptr;
+        HIP_TRY(hipStreamEndCapture(s->stream, &g));
+        if (rc != CUPSS_OK) {
+            hipGraphDestroy(g);
+            return rc;
+        }
+        s->graph = g;
+        HIP_TRY(hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0));
+        s->graph_gen = s->gen;
+        s->graph_stream = s->stream;
+        HIP_TRY(hipGraphLaunch(s->gexec, s->stream));
+        done = K;
+    }
+    while (nsteps - done >= K) {
+        HIP_TRY(hipGraphLaunch(s->gexec, s->stream));
+        s->step += (uint64_t)K;
+        done += K;
+    }
+    return general_steps(s, nsteps - done);
+}
+
 
 extern "C" int cupss_system_advance(cupss_system s, int nsteps)
 {
@@ -607,7 +675,36 @@ extern "C" int cupss_system_advance(cupss_system s, int nsteps)
         }
         return fused_steps(s, nsteps);
     }
-    return general_steps(s, nsteps);
+    return general_advance(s, nsteps);
+}
+
+extern "C" int cupss_system_set_graph_steps(cupss_system s, int k)
+{
+    if (int r = check_sys(s)) return r;
+    if (k != 0 && (k < 2 || (k & 1))) return fail(CUPSS_ERR_INVALID, "graph steps must be 0 or an even number >= 2");
+    s->graph_steps = k;
+    drop_general_graph(s);
+    return CUPSS_OK;
+}
+
+extern "C" int cupss_system_device_bytes(cupss_system s, size_t *bytes)
+{
+    if (int r = check_sys(s)) return r;
+    if (!bytes) return fail(CUPSS_ERR_INVALID, "null bytes");
+    const size_t spec = sizeof(float2) * (size_t)s->g.nh, real = sizeof(float) * (size_t)s->g.n;
+    size_t b = spec + real;  // scratch, rtmp
+    if (s->full) b += sizeof(float2) * (size_t)s->g.n;
+    for (auto &F : s->f) {
+        b += 2 * spec;
+        if (F.dreal) b += real;
+        if (F.eta) b += spec;
+    }
+    for (auto &T : s->t)
+        if (T.xhat) b += spec;
+    if (s->plan.ok)  // the fused path's buffers, allocated at its first run
+        b += spec * (2 + s->plan.real.size() + s->plan.groups.size());
+    *bytes = b;
+    return CUPSS_OK;
 }
 
 extern "C" int cupss_system_set_path(cupss_system s, int mode)

@@ -81,6 +81,15 @@ struct cupss_system_t {
     int device = 0;
     void *batch = nullptr;    // internal one-instance fused batch
     uint64_t gen = 1;         // bumped by every change that alters the step program
+    // general path: K steps captured once in a hipGraph and replayed
+    // (noiseless systems without hooks; K even so every field's spectrum
+    // returns to its first buffer)
+    int graph_steps = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    uint64_t graph_gen = 0;
+    hipStream_t graph_stream = nullptr;
+    std::vector<int> graph_cur;  // each field's buffer parity the graph was captured at
 };
 
 namespace cupss {

@@ -175,6 +175,20 @@ __device__ __forceinline__ bool keep_mode(const Geom &g, int p, int kx, int ky, 
 // self-conjugate bins.  Bins with kx = 0 or nc have their partner
 // (kx, -ky, -kz) inside the half spectrum: the partner with the smaller
 // (kz, ky) draws, the other takes the conjugate.
+//
+// Draws are paired along y: bins (kx, j, kz) and (kx, j + hy, kz), with
+// hy = sy/2 (1 if sy = 1) and j < hy, share one Philox4x32-10 call with
+// counter (pair index, step_lo, step_hi, stream | 0x80000000); outputs 0,1
+// give the first bin's Box-Muller pair and outputs 2,3 the second's.
+__device__ __forceinline__ int noise_hy(const Geom &g) { return g.sy >= 2 ? g.sy / 2 : 1; }
+
+__device__ __forceinline__ Philox4 noise_philox(const Geom &g, int kx, int j, int kz, uint32_t slo, uint32_t shi,
+                                                uint32_t stream, uint32_t k0, uint32_t k1)
+{
+    const uint32_t pair = (uint32_t)(((long)kz * g.nc1 + kx) * noise_hy(g) + j);
+    return philox4x32_10(pair, slo, shi, stream | 0x80000000u, k0, k1);
+}
+
 __device__ __forceinline__ float2 spectral_noise(const Geom &g, int kx, int ky, int kz, uint32_t slo, uint32_t shi,
                                                  uint32_t stream, uint32_t k0, uint32_t k1)
 {
@@ -190,13 +204,32 @@ __device__ __forceinline__ float2 spectral_noise(const Geom &g, int kx, int ky, 
             conj = true;
         }
     }
-    const uint32_t hc = (uint32_t)(((long)cz * g.sy + cy) * g.nc1 + kx);
-    const Philox4 o = philox4x32_10(hc, slo, shi, stream | 0x80000000u, k0, k1);
-    const float2 z = box_muller(o.x[0], o.x[1]);
+    const int hy = noise_hy(g);
+    const Philox4 o = noise_philox(g, kx, cy % hy, cz, slo, shi, stream, k0, k1);
+    const float2 z = (cy / hy) ? box_muller(o.x[2], o.x[3]) : box_muller(o.x[0], o.x[1]);
     const float n = (float)g.n;
     if (real) return make_float2(sqrtf(n) * z.x, 0.0f);
     const float s = sqrtf(0.5f * n);
     return make_float2(s * z.x, conj ? -s * z.y : s * z.y);
+}
+
+// Both bins of the pair (kx, j, kz), (kx, j + hy, kz) from one Philox call
+// (sy >= 2); equal to spectral_noise() of each bin.
+__device__ __forceinline__ void spectral_noise_pair(const Geom &g, int kx, int j, int kz, uint32_t slo, uint32_t shi,
+                                                    uint32_t stream, uint32_t k0, uint32_t k1, float2 &lo,
+                                                    float2 &hi)
+{
+    if (kx == 0 || kx == g.nc) {
+        lo = spectral_noise(g, kx, j, kz, slo, shi, stream, k0, k1);
+        hi = spectral_noise(g, kx, j + noise_hy(g), kz, slo, shi, stream, k0, k1);
+        return;
+    }
+    const Philox4 o = noise_philox(g, kx, j, kz, slo, shi, stream, k0, k1);
+    const float2 z0 = box_muller(o.x[0], o.x[1]);
+    const float2 z1 = box_muller(o.x[2], o.x[3]);
+    const float s = sqrtf(0.5f * (float)g.n);
+    lo = make_float2(s * z0.x, s * z0.y);
+    hi = make_float2(s * z1.x, s * z1.y);
 }
 
 }  // namespace cupss

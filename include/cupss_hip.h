@@ -143,6 +143,15 @@ int cupss_system_set_path(cupss_system s, int mode);
 int cupss_system_set_noise_mode(cupss_system s, int mode);
 /* 1 if cupss_system_advance would take the fused step */
 int cupss_system_fused_eligible(cupss_system s);
+/* 0 = direct launches (default); K (even, >= 2) = runs of at least K steps
+ * replay K steps from a hipGraph captured at the first such run.  Applies
+ * to the fused path and to the general step program of noiseless systems
+ * without callbacks (others keep direct launches).  No counterpart in the
+ * reference (src/evolver.cpp:199-226 launches every kernel of every step). */
+int cupss_system_set_graph_steps(cupss_system s, int k);
+/* device bytes the system holds, including the fused path's buffers it
+ * allocates at its first fused run (for admission by memory budget) */
+int cupss_system_device_bytes(cupss_system s, size_t *bytes);
 /* evolver::dt change is not part of the reference API; exposed for sweeps */
 int cupss_system_set_dt(cupss_system s, float dt);
 
@@ -160,6 +169,9 @@ typedef struct cupss_batch_t *cupss_batch;
 int cupss_batch_create(void *hip_stream, cupss_batch *out);
 int cupss_batch_destroy(cupss_batch b);
 int cupss_batch_add(cupss_batch b, cupss_system s);
+/* take a system out of the batch (it keeps its state and can be advanced
+ * on its own again) */
+int cupss_batch_remove(cupss_batch b, cupss_system s);
 int cupss_batch_size(cupss_batch b);
 /* 0 = direct launches; K (even, >= 2) = capture K steps in a hipGraph */
 int cupss_batch_set_graph_steps(cupss_batch b, int k);
@@ -173,6 +185,35 @@ int cupss_batch_schedule(cupss_batch b, char *buf, int len);
 int cupss_batch_advance(cupss_batch b, int nsteps);
 int cupss_batch_time_advance(cupss_batch b, int nsteps, float *ms);
 int cupss_batch_synchronize(cupss_batch b);
+
+/* ---- serving -----------------------------------------------------------
+ * Jobs (systems with a step count) arrive over time and are served with
+ * continuous batching: each round admits the jobs that have arrived, in
+ * arrival order, while the head of the queue fits `budget_bytes` of device
+ * memory and fewer than `max_active` jobs are resident (an idle device
+ * always takes the head); then every resident job advances by up to
+ * `quantum` steps -- jobs that qualify for the fused step together in one
+ * batch, the others on their own streams -- and finished jobs are handed
+ * back and freed.  A job's system is built only at admission, by the
+ * caller's `build` callback (which returns a prepared system, e.g. built
+ * through the parser); `done` is called with the finished system just
+ * before it is destroyed, to read results.  A job's result is the same as
+ * advancing its system alone.  No counterpart in the reference. */
+typedef struct cupss_server_t *cupss_server;
+typedef int (*cupss_job_build)(void *user, int job, cupss_system *out);
+typedef void (*cupss_job_done)(void *user, int job, cupss_system s);
+int cupss_server_create(size_t budget_bytes, int quantum, int max_active, cupss_server *out);
+int cupss_server_destroy(cupss_server srv);
+/* arrival: seconds after cupss_server_run starts; bytes_estimate: device
+ * memory the job will hold (cupss_system_device_bytes of a like system) */
+int cupss_server_submit(cupss_server srv, double arrival, int nsteps, size_t bytes_estimate,
+                        cupss_job_build build, cupss_job_done done, void *user, int *job);
+/* serve every submitted job to completion (wall-clock arrivals) */
+int cupss_server_run(cupss_server srv);
+/* seconds after the start of the run; -1 where not reached */
+int cupss_server_job_times(cupss_server srv, int job, double *arrival, double *admitted, double *finished);
+int cupss_server_stats(cupss_server srv, double *wall_s, double *point_steps, size_t *peak_bytes,
+                       int *peak_active);
 
 /* Device buffer access for callers that bring their own kernels (e.g. the
  * reference's user callbacks that launch __global__ functions): device

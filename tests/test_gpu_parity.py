@@ -102,7 +102,7 @@ def test_cahn_hilliard_2d(n, path):
 
 @pytest.mark.parametrize("path", [0, 1])
 def test_cahn_hilliard_2d_1024_parity(path):
-    """Config 2 at its full size, 100 steps."""
+    """2D Cahn-Hilliard at 1024^2, 100 steps."""
     compare(spec_cahn_hilliard((1024, 1024, 1)), 100, path=path)
 
 
