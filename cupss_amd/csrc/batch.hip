@@ -183,6 +183,7 @@ static int describe(cupss_system s, FusedInst &a)
         const int f = P.real[i];
         a.p[i] = s->f[f].order;
         a.kmax[i] = a.p[i] > 0 ? std::min(g.nc, g.sx / (a.p[i] + 1)) : g.nc;
+        a.kymax[i] = a.p[i] > 0 ? g.sy / (a.p[i] + 1) : g.sy;
         a.Z[i] = s->fz[i];
         a.zero[i] = (f == P.phi && F.dstate == 0) ? 1 : 0;
         if (f != P.phi) {
@@ -209,9 +210,8 @@ static int describe(cupss_system s, FusedInst &a)
         a.g0[gi] = np;
         for (int ti : P.groups[gi]) {
             const TermRec &T = s->t[ti];
-            a.nfac[np] = (int)T.fac.size();
-            for (size_t c = 0; c < T.fac.size(); c++)
-                a.fac[np][c] = (int)(std::find(P.real.begin(), P.real.end(), T.fac[c]) - P.real.begin());
+            for (int c : T.fac)
+                a.cnt[np][std::find(P.real.begin(), P.real.end(), c) - P.real.begin()]++;
             np++;
         }
     }

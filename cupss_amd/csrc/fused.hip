@@ -185,16 +185,17 @@ __global__ __launch_bounds__((FRowCfg<NC, LPB>::THREADS)) void k_frows(const Fus
             float2 p[E];
 #pragma unroll
             for (int e = 0; e < E; e++) p[e] = make_float2(1.f, 1.f);
-            for (int c = 0; c < a.nfac[pi]; c++) {
-                const int f = a.fac[pi][c];
+            // product = prod_f r_f^cnt_f: static field index, uniform powers
+            // (no dynamic indexing, so r stays in registers)
 #pragma unroll
-                for (int e = 0; e < E; e++) {
-                    // select without dynamic indexing (keeps r in registers)
-                    const float2 x = f == 0 ? r[0][e] : (f == 1 ? r[1][e] : r[2][e]);
-                    p[e].x *= x.x;
-                    p[e].y *= x.y;
+            for (int f = 0; f < FUSED_MAX_REAL; f++)
+                for (int c = 0; c < a.cnt[pi][f]; c++) {
+#pragma unroll
+                    for (int e = 0; e < E; e++) {
+                        p[e].x *= r[f][e].x;
+                        p[e].y *= r[f][e].y;
+                    }
                 }
-            }
 #pragma unroll
             for (int e = 0; e < E; e++) v[e] = cadd(v[e], p[e]);
         }
@@ -273,9 +274,13 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
     load_twiddles<N>(tws, sets);
     __syncthreads();
 
+    ColBins<E> cb;
+    col_bins<E, T>(g, kx, t, cb);
     if (upd) {
+        // phi is the dynamic field (setDynamic_k, src/field_kernels.cu:199-227)
         const uint64_t step = a.step_base + step_rel;
         float2 acc[E];
+        float m[E];
 #pragma unroll
         for (int e = 0; e < E; e++) acc[e] = make_float2(0.f, 0.f);
         for (int gi = 0; gi < ngroups; gi++) {
@@ -285,12 +290,9 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
             }
             fft_line<N, E, -1, N>(xv, L, t, tws);
             const TermRefDev &TR = a.u.terms[a.gterm[gi]];
+            multiplier_col<E>(a.u.pres, TR.pres0, TR.npres, TR.mult_i, cb, m);
 #pragma unroll
-            for (int e = 0; e < E; e++) {
-                const Bin b = make_bin(g, kx, t + e * T, 0);
-                acc[e] = cadd(acc[e],
-                              apply_mult(multiplier(a.u.pres, TR.pres0, TR.npres, TR.mult_i, b), TR.mult_i, xv[e]));
-            }
+            for (int e = 0; e < E; e++) acc[e] = cadd(acc[e], apply_mult(m[e], TR.mult_i, xv[e]));
             __syncthreads();
         }
         if (a.noise_mode == NOISE_REAL) {
@@ -303,20 +305,36 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
                 spectral_noise_pair(g, kx, t + e * T, 0, (uint32_t)step, (uint32_t)(step >> 32), a.stream, a.key0,
                                     a.key1, eta[e], eta[e + E / 2]);
         }
+        // the remaining terms are constants or linear in phi itself
+        for (int ti = 0; ti < a.u.nterms; ti++) {
+            const TermRefDev &TR = a.u.terms[ti];
+            if (TR.kind == TERM_GROUPED) continue;
+            multiplier_col<E>(a.u.pres, TR.pres0, TR.npres, TR.mult_i, cb, m);
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const float2 X = TR.kind == TERM_CONST
+                                     ? (cb.zero[e] ? make_float2((float)g.n, 0.f) : make_float2(0.f, 0.f))
+                                     : cur[e];
+                acc[e] = cadd(acc[e], apply_mult(m[e], TR.mult_i, X));
+            }
+        }
+        float rD[E];
+        if (a.u.nimp > 0) implicit_col<E>(a.u, cb, rD);
 #pragma unroll
         for (int e = 0; e < E; e++) {
-            const Bin b = make_bin(g, kx, t + e * T, 0);
-            float2 s = acc[e];
-            for (int ti = 0; ti < a.u.nterms; ti++) {
-                const TermRefDev &TR = a.u.terms[ti];
-                if (TR.kind == TERM_GROUPED) continue;
-                // the remaining terms are constants or linear in phi itself
-                const float2 X = TR.kind == TERM_CONST
-                                     ? (b.h == 0 ? make_float2((float)g.n, 0.f) : make_float2(0.f, 0.f))
-                                     : cur[e];
-                s = cadd(s, apply_mult(multiplier(a.u.pres, TR.pres0, TR.npres, TR.mult_i, b), TR.mult_i, X));
+            float2 s = cur[e];
+            s.x += a.u.dt * acc[e].x;
+            s.y += a.u.dt * acc[e].y;
+            if (a.u.noisy) {
+                const float pn = noise_amplitude_col<E>(a.u, cb, e);
+                s.x += pn * eta[e].x;
+                s.y += pn * eta[e].y;
             }
-            cur[e] = finish_update(a.u, b, cur[e], s, eta[e]);
+            if (a.u.nimp > 0) {
+                s.x *= rD[e];
+                s.y *= rD[e];
+            }
+            cur[e] = s;
         }
         if (active) {
             float2 *__restrict__ sout = a.spec[parity ^ 1];
@@ -330,17 +348,17 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
         if (f < a.nreal && blk.line <= a.kmax[f]) {
             float2 v[E];
             const bool zero = !upd && a.zero[f];
+            float lm[E];
+            const TermRefDev &LM = a.lmul[f];
+            if (LM.npres > 0) multiplier_col<E>(a.lpres, LM.pres0, LM.npres, LM.mult_i, cb, lm);
 #pragma unroll
             for (int e = 0; e < E; e++) {
                 const int ky = t + e * T;
                 float2 w = cur[e];
-                if (zero || !keep_mode(g, a.p[f], kx, ky, 0)) {
+                if (zero || !keep_mode2(g, a.kmax[f], a.kymax[f], kx, ky))
                     w = make_float2(0.f, 0.f);
-                } else if (a.lmul[f].npres > 0) {
-                    const Bin b = make_bin(g, kx, ky, 0);
-                    const TermRefDev &LM = a.lmul[f];
-                    w = apply_mult(multiplier(a.lpres, LM.pres0, LM.npres, LM.mult_i, b), LM.mult_i, w);
-                }
+                else if (LM.npres > 0)
+                    w = apply_mult(lm[e], LM.mult_i, w);
                 v[e] = w;
             }
             fft_line<N, E, +1, N>(v, L, t, tws);

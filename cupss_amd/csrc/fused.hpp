@@ -45,8 +45,7 @@ struct FusedInst {
     int nreal;
     int ngroups;
     int g0[FUSED_MAX_GROUPS + 1];             // products of group g: [g0[g], g0[g+1])
-    int nfac[FUSED_MAX_PRODS];
-    int fac[FUSED_MAX_PRODS][FUSED_MAX_FAC];  // indices into Z / the real fields
+    int cnt[FUSED_MAX_PRODS][FUSED_MAX_REAL]; // power of each real field in product p
     int noise_mode;
     uint32_t key0, key1, stream;
     // column pass
@@ -54,6 +53,7 @@ struct FusedInst {
     int gterm[FUSED_MAX_GROUPS];              // u.terms index carrying group g's prefactors
     int p[FUSED_MAX_REAL];                    // dealias order of each real field
     int kmax[FUSED_MAX_REAL];                 // last column kx the dealias mask of field f keeps
+    int kymax[FUSED_MAX_REAL];                // largest |ky| the mask keeps
     int zero[FUSED_MAX_REAL];                 // real field is zero in the prologue (step-0 state)
     TermRefDev lmul[FUSED_MAX_REAL];          // L_f (npres = 0: identity)
     PresDev lpres[FUSED_MAX_LPRES];

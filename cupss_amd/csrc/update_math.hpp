@@ -46,7 +46,8 @@ __device__ __forceinline__ Bin make_bin(const Geom &g, int kx, int ky, int kz)
     b.nyx = g.sx > 1 && kx == g.sx / 2;
     b.nyy = g.sy > 1 && ky == g.sy / 2;
     b.nyz = g.sz > 1 && kz == g.sz / 2;
-    const float r = 1.0f / sqrtf(b.q2);
+    // hardware reciprocal square root (q2 = 0 gives +inf, masked below)
+    const float r = __builtin_amdgcn_rsqf(b.q2);
     b.iq = b.h > 0 ? r : 0.0f;
     b.iqd = (kx > 0 || ky > 0) ? r : 0.0f;
     return b;
@@ -56,7 +57,9 @@ __device__ __forceinline__ Bin make_bin(const Geom &g, int kx, int ky, int kz)
 // monomial i^m c q^2n qx^a qy^b qz^c |q|^-k survives the reference's
 // IFFT/.y=0/FFT round trip iff m + (sum of its q_d powers over the axes
 // where this bin is not a Nyquist index) is even; otherwise its image
-// cancels against its Hermitian partner and it contributes nothing.
+// cancels against its Hermitian partner and it contributes nothing.  The
+// parity test is a per-bin select (no divergent branch); the exponent
+// loops have uniform trip counts.
 __device__ __forceinline__ float multiplier(const PresDev *__restrict__ pres, int p0, int np, int mult_i,
                                             const Bin &b)
 {
@@ -64,8 +67,7 @@ __device__ __forceinline__ float multiplier(const PresDev *__restrict__ pres, in
     for (int p = p0; p < p0 + np; p++) {
         const PresDev P = pres[p];
         const int par = mult_i + (b.nyx ? 0 : P.a) + (b.nyy ? 0 : P.b) + (b.nyz ? 0 : P.cz);
-        if (par & 1) continue;
-        float v = P.c;
+        float v = (par & 1) ? 0.0f : P.c;
         if (P.q2n > 0) v *= ipow(b.q2, P.q2n);
         if (P.a > 0) v *= ipow(b.qx, P.a);
         if (P.b > 0) v *= ipow(b.qy, P.b);
@@ -128,8 +130,9 @@ __device__ __forceinline__ float2 finish_update(const UpdateArgs &a, const Bin &
                 if (P.invq != 0) th *= ipow(b.iqd, P.invq);
                 D -= a.dt * th;
             }
-            s.x /= D;
-            s.y /= D;
+            const float rD = __builtin_amdgcn_rcpf(D);  // hardware reciprocal, once per bin
+            s.x *= rD;
+            s.y *= rD;
         }
     } else {
         s = a.nterms > 0 ? acc : cur;
@@ -152,14 +155,134 @@ __device__ __forceinline__ float2 finish_update(const UpdateArgs &a, const Bin &
                 if (P.invq != 0) th *= ipow(b.iq, P.invq);
                 L += th;
             }
-            s.x /= L;
-            s.y /= L;
+            const float rL = __builtin_amdgcn_rcpf(L);
+            s.x *= rL;
+            s.y *= rL;
         }
     }
     return s;
 }
 
+// ---- the same arithmetic for the E bins (kx, t + e*T), e < E, that one
+// lane of a column kernel holds (2D instances).  Loops over prefactors run
+// once per lane with all E bins inside them, so every prefactor is loaded
+// once and the loop overhead is shared.
+
+template <int E>
+struct ColBins {
+    int kx;
+    float qx;
+    bool nyx;
+    float qy[E], q2[E];
+    bool nyy[E], zero[E];  // Nyquist row; the q = 0 bin
+};
+
+template <int E, int T>
+__device__ __forceinline__ void col_bins(const Geom &g, int kx, int t, ColBins<E> &cb)
+{
+    cb.kx = kx;
+    cb.qx = (float)(kx < (g.sx + 1) / 2 ? kx : kx - g.sx) * g.sqx;
+    cb.nyx = g.sx > 1 && kx == g.sx / 2;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int ky = t + e * T;
+        cb.qy[e] = (float)(ky < (g.sy + 1) / 2 ? ky : ky - g.sy) * g.sqy;
+        cb.q2[e] = cb.qx * cb.qx + cb.qy[e] * cb.qy[e];
+        cb.nyy[e] = g.sy > 1 && ky == g.sy / 2;
+        cb.zero[e] = kx == 0 && ky == 0;
+    }
+}
+
+// 1/|q| of bin e (0 at q = 0; in 2D the i = j = 0 line is that bin alone)
+template <int E>
+__device__ __forceinline__ float col_iq(const ColBins<E> &cb, int e)
+{
+    return cb.zero[e] ? 0.0f : __builtin_amdgcn_rsqf(cb.q2[e]);
+}
+
+// multiplier() for the E bins of a column
+template <int E>
+__device__ __forceinline__ void multiplier_col(const PresDev *__restrict__ pres, int p0, int np, int mult_i,
+                                               const ColBins<E> &cb, float (&m)[E])
+{
+#pragma unroll
+    for (int e = 0; e < E; e++) m[e] = 0.0f;
+    for (int p = p0; p < p0 + np; p++) {
+        const PresDev P = pres[p];
+        if (P.cz > 0) continue;  // qz = 0 in 2D
+        float base = P.c;
+        for (int i = 0; i < P.a; i++) base *= cb.qx;
+        const int parx = mult_i + (cb.nyx ? 0 : P.a);
+        float v[E];
+#pragma unroll
+        for (int e = 0; e < E; e++) v[e] = base;
+        for (int i = 0; i < P.q2n; i++)
+#pragma unroll
+            for (int e = 0; e < E; e++) v[e] *= cb.q2[e];
+        for (int i = 0; i < P.b; i++)
+#pragma unroll
+            for (int e = 0; e < E; e++) v[e] *= cb.qy[e];
+        if (P.invq > 0) {
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const float iq = col_iq(cb, e);
+                for (int i = 0; i < P.invq; i++) v[e] *= iq;
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int par = parx + (cb.nyy[e] ? 0 : P.b);
+            m[e] += (par & 1) ? 0.0f : v[e];
+        }
+    }
+}
+
+// D(q) = 1 - dt * sum_imp (setDynamic_k) and the noise amplitude, per bin
+template <int E>
+__device__ __forceinline__ void implicit_col(const UpdateArgs &a, const ColBins<E> &cb, float (&rD)[E])
+{
+    float D[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) D[e] = 1.0f;
+    for (int i = 0; i < a.nimp; i++) {
+        const PresDev P = a.imp[i];
+        float th[E];
+#pragma unroll
+        for (int e = 0; e < E; e++) th[e] = P.c;
+        for (int k = 0; k < P.q2n; k++)
+#pragma unroll
+            for (int e = 0; e < E; e++) th[e] *= cb.q2[e];
+        if (P.invq != 0) {
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const float iq = col_iq(cb, e);
+                for (int k = 0; k < P.invq; k++) th[e] *= iq;
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < E; e++) D[e] -= a.dt * th[e];
+    }
+#pragma unroll
+    for (int e = 0; e < E; e++) rD[e] = __builtin_amdgcn_rcpf(D[e]);
+}
+
+template <int E>
+__device__ __forceinline__ float noise_amplitude_col(const UpdateArgs &a, const ColBins<E> &cb, int e)
+{
+    float pn = a.noise_base;
+    if (a.noise_q2n != 0) pn *= sqrtf(ipow(cb.q2[e], a.noise_q2n));
+    if (a.noise_invq != 0) pn *= sqrtf(ipow(col_iq(cb, e), a.noise_invq));
+    return pn;
+}
+
 // dealias_k (src/field_kernels.cu:229-256): keep |n_d| <= s_d/(p+1) on every axis
+// (limits precomputed: lx = sx/(p+1), ly = sy/(p+1), 2D)
+__device__ __forceinline__ bool keep_mode2(const Geom &g, int lx, int ly, int kx, int ky)
+{
+    const int ny = ky > g.sy / 2 ? ky - g.sy : ky;
+    return kx <= lx && abs(ny) <= ly;
+}
+
 __device__ __forceinline__ bool keep_mode(const Geom &g, int p, int kx, int ky, int kz)
 {
     if (p <= 0) return true;
@@ -208,8 +331,8 @@ __device__ __forceinline__ float2 spectral_noise(const Geom &g, int kx, int ky, 
     const Philox4 o = noise_philox(g, kx, cy % hy, cz, slo, shi, stream, k0, k1);
     const float2 z = (cy / hy) ? box_muller(o.x[2], o.x[3]) : box_muller(o.x[0], o.x[1]);
     const float n = (float)g.n;
-    if (real) return make_float2(sqrtf(n) * z.x, 0.0f);
-    const float s = sqrtf(0.5f * n);
+    if (real) return make_float2(__builtin_amdgcn_sqrtf(n) * z.x, 0.0f);
+    const float s = __builtin_amdgcn_sqrtf(0.5f * n);
     return make_float2(s * z.x, conj ? -s * z.y : s * z.y);
 }
 
@@ -227,7 +350,7 @@ __device__ __forceinline__ void spectral_noise_pair(const Geom &g, int kx, int j
     const Philox4 o = noise_philox(g, kx, j, kz, slo, shi, stream, k0, k1);
     const float2 z0 = box_muller(o.x[0], o.x[1]);
     const float2 z1 = box_muller(o.x[2], o.x[3]);
-    const float s = sqrtf(0.5f * (float)g.n);
+    const float s = __builtin_amdgcn_sqrtf(0.5f * (float)g.n);
     lo = make_float2(s * z0.x, s * z0.y);
     hi = make_float2(s * z1.x, s * z1.y);
 }
