@@ -61,6 +61,7 @@ struct FusedBatch {
     struct Class {
         int nc = 0, ny = 0;
         bool real = false;      // any instance has real fields (prologue)
+        bool wide = false;      // row kernel for more than 3 real fields
         std::vector<int> insts;
         int lpb = 0, twd = 0;   // tile shapes
         std::vector<Tables> chunks;
@@ -143,11 +144,43 @@ struct FusedBatch {
     }
 };
 
-static bool is_elementwise(const FusedInst &a) { return a.ngroups == 0 && a.nreal == 0 && a.noise_mode != NOISE_REAL; }
+// no transform needed: no products, no stored fields, no real-space noise,
+// and phi's other terms read phi itself or nothing
+static bool is_elementwise(const FusedInst &a)
+{
+    if (a.ngroups != 0 || a.nreal != 0 || a.ncons != 0 || a.noise_mode == NOISE_REAL) return false;
+    for (int i = 0; i < a.tgt[0].nterms; i++)
+        if (a.tgt[0].term[i].src != SRC_OLD && a.tgt[0].term[i].src != SRC_CONST) return false;
+    return true;
+}
 
 static int alloc_spec(float2 **p, const Geom &g)
 {
     if (!*p && hipMalloc(p, sizeof(float2) * g.nh) != hipSuccess) return fail(CUPSS_ERR_NOMEM, "device allocation failed");
+    return CUPSS_OK;
+}
+
+// prefactors of one term multiplier into the pool (S2, src/term_init.cpp:
+// 135-200: i^m factored out, the sign of each monomial's i^(I-m) folded in)
+static int pool_term(FusedInst &a, int &np, const std::vector<cupss_pres> &pres, TermRefDev &R)
+{
+    const int m = pres.empty() ? 0 : (pres[0].iqx + pres[0].iqy + pres[0].iqz) % 2;
+    R.X = nullptr;
+    R.kind = 0;
+    R.mult_i = m;
+    R.pres0 = np;
+    R.npres = (int)pres.size();
+    for (const cupss_pres &p : pres) {
+        if (np >= FUSED_MAX_POOL) return fail(CUPSS_ERR_UNSUPPORTED, "too many prefactors for the fused step");
+        const int iu = p.iqx + p.iqy + p.iqz;
+        PresDev &D = a.pool[np++];
+        D.c = p.preFactor * (float)(-2 * (((iu - m) / 2) % 2) + 1);
+        D.q2n = p.q2n;
+        D.a = p.iqx;
+        D.b = p.iqy;
+        D.cz = p.iqz;
+        D.invq = p.invq;
+    }
     return CUPSS_OK;
 }
 
@@ -165,20 +198,91 @@ static int describe(cupss_system s, FusedInst &a)
     a.g = g;
     a.eta = F.eta;
     a.step_base = s->step;
+    a.scale = 1.0f / (float)g.n;
     build_update(s, P.phi, a.u);
     a.u.noise = F.eta;
+    a.noise_mode = !F.noisy ? NOISE_NONE : (s->noise_mode == 1 ? NOISE_SPECTRAL : NOISE_REAL);
+    a.key0 = (uint32_t)F.seed;
+    a.key1 = (uint32_t)(F.seed >> 32);
+    a.stream = (uint32_t)P.phi;
+    int np = 0;
+    auto index_of = [](const std::vector<int> &v, int x) { return (int)(std::find(v.begin(), v.end(), x) - v.begin()); };
+    // derived fields: one multiplier each (also L_f of their real fields)
+    a.nder = (int)P.derived.size();
+    for (size_t d = 0; d < P.derived.size(); d++)
+        if (int r = pool_term(a, np, s->t[s->f[P.derived[d]].terms[0]].pres, a.dmul[d])) return r;
+    // stored constraint fields
+    a.ncons = (int)P.cons.size();
+    // targets: phi, then the stored fields; groups are ordered the same way
+    std::vector<int> targets = {P.phi};
+    targets.insert(targets.end(), P.cons.begin(), P.cons.end());
     a.ngroups = (int)P.groups.size();
-    for (size_t gi = 0; gi < P.groups.size(); gi++) {
-        a.X[gi] = s->fx[gi];
-        for (size_t k = 0; k < F.terms.size(); k++)
-            for (int ti : P.groups[gi])
-                if (F.terms[k] == ti) {
-                    a.u.terms[k].kind = TERM_GROUPED;
-                    if (ti == P.groups[gi][0]) a.gterm[gi] = (int)k;
-                }
+    int gi = 0, pi = 0;
+    for (size_t k = 0; k < targets.size(); k++) {
+        const int f = targets[k];
+        FusedTarget &TG = a.tgt[k];
+        TG.g0 = gi;
+        while (gi < (int)P.groups.size() && P.groups[gi].target == f) {
+            const FusedGroup &grp = P.groups[gi];
+            a.X[gi] = s->fx[gi];
+            a.pg0[gi] = pi;
+            for (size_t q = 0; q < grp.terms.size(); q++) {
+                for (int c : s->t[grp.terms[q]].fac) a.cnt[pi][index_of(P.real, c)]++;
+                a.coef[pi] = grp.coef[q];
+                pi++;
+            }
+            if (int r = pool_term(a, np, grp.pres, a.gmul[gi])) return r;
+            gi++;
+        }
+        TG.g1 = gi;
+        TG.nterms = 0;
+        for (int ti : s->f[f].terms) {
+            const TermRec &T = s->t[ti];
+            if (T.fac.size() >= 2) continue;
+            FusedTerm &FT = TG.term[TG.nterms++];
+            TermRefDev R;
+            if (int r = pool_term(a, np, T.pres, R)) return r;
+            FT.mult_i = R.mult_i;
+            FT.pres0 = R.pres0;
+            FT.npres = R.npres;
+            FT.idx = 0;
+            if (T.fac.empty()) {
+                FT.src = SRC_CONST;
+            } else if (T.fac[0] == P.phi) {
+                FT.src = k == 0 ? SRC_OLD : SRC_NEW;
+            } else if (std::find(P.derived.begin(), P.derived.end(), T.fac[0]) != P.derived.end()) {
+                FT.src = SRC_DERIVED;
+                FT.idx = index_of(P.derived, T.fac[0]);
+            } else {
+                FT.src = SRC_CONS;
+                FT.idx = index_of(P.cons, T.fac[0]);
+            }
+        }
+        TG.nimp = 0;
+        TG.imp0 = np;
+        TG.z = -1;
+        if (k > 0) {
+            for (const cupss_pres &q : s->f[f].imp) {
+                if (np >= FUSED_MAX_POOL) return fail(CUPSS_ERR_UNSUPPORTED, "too many prefactors for the fused step");
+                PresDev &D = a.pool[np++];
+                D.c = q.preFactor;
+                D.q2n = q.q2n;
+                D.a = q.iqx;
+                D.b = q.iqy;
+                D.cz = q.iqz;
+                D.invq = q.invq;
+                TG.nimp++;
+            }
+            const int c = (int)k - 1;
+            if (int r = alloc_spec(&s->fc[c][0], g)) return r;
+            if (int r = alloc_spec(&s->fc[c][1], g)) return r;
+            a.cs[c][0] = s->fc[c][0];
+            a.cs[c][1] = s->fc[c][1];
+        }
     }
+    a.pg0[gi] = pi;
+    // real fields
     a.nreal = (int)P.real.size();
-    int lp = 0;
     for (size_t i = 0; i < P.real.size(); i++) {
         const int f = P.real[i];
         a.p[i] = s->f[f].order;
@@ -186,62 +290,16 @@ static int describe(cupss_system s, FusedInst &a)
         a.kymax[i] = a.p[i] > 0 ? g.sy / (a.p[i] + 1) : g.sy;
         a.Z[i] = s->fz[i];
         a.zero[i] = (f == P.phi && F.dstate == 0) ? 1 : 0;
-        if (f != P.phi) {
-            const TermRec &T = s->t[s->f[f].terms[0]];
-            const int m = (T.pres[0].iqx + T.pres[0].iqy + T.pres[0].iqz) % 2;
-            a.lmul[i].mult_i = m;
-            a.lmul[i].pres0 = lp;
-            a.lmul[i].npres = (int)T.pres.size();
-            for (const cupss_pres &p : T.pres) {
-                const int iu = p.iqx + p.iqy + p.iqz;
-                PresDev &D = a.lpres[lp++];
-                D.c = p.preFactor * (float)(-2 * (((iu - m) / 2) % 2) + 1);
-                D.q2n = p.q2n;
-                D.a = p.iqx;
-                D.b = p.iqy;
-                D.cz = p.iqz;
-                D.invq = p.invq;
-            }
+        a.zsrc[i] = -1;
+        a.lmul[i] = TermRefDev{nullptr, 0, 0, 0, 0};
+        const int d = index_of(P.derived, f), c = index_of(P.cons, f);
+        if (d < (int)P.derived.size()) {
+            a.lmul[i] = a.dmul[d];
+        } else if (c < (int)P.cons.size()) {
+            a.zsrc[i] = c;
+            a.tgt[1 + c].z = (int)i;
         }
     }
-    a.scale = 1.0f / (float)g.n;
-    int np = 0;
-    for (size_t gi = 0; gi < P.groups.size(); gi++) {
-        a.g0[gi] = np;
-        for (int ti : P.groups[gi]) {
-            const TermRec &T = s->t[ti];
-            for (int c : T.fac)
-                a.cnt[np][std::find(P.real.begin(), P.real.end(), c) - P.real.begin()]++;
-            np++;
-        }
-    }
-    a.g0[P.groups.size()] = np;
-    a.noise_mode = !F.noisy ? NOISE_NONE : (s->noise_mode == 1 ? NOISE_SPECTRAL : NOISE_REAL);
-    if ((int)P.derived.size() > FUSED_MAX_DERIVED) return fail(CUPSS_ERR_UNSUPPORTED, "too many derived fields");
-    int dp = 0;
-    a.nder = (int)P.derived.size();
-    for (size_t d = 0; d < P.derived.size(); d++) {
-        const FieldRec &D = s->f[P.derived[d]];
-        const TermRec &T = s->t[D.terms[0]];
-        const int m = (T.pres[0].iqx + T.pres[0].iqy + T.pres[0].iqz) % 2;
-        a.dmul[d].mult_i = m;
-        a.dmul[d].pres0 = dp;
-        a.dmul[d].npres = (int)T.pres.size();
-        for (const cupss_pres &p : T.pres) {
-            if (dp >= FUSED_MAX_LPRES) return fail(CUPSS_ERR_UNSUPPORTED, "too many derived prefactors");
-            const int iu = p.iqx + p.iqy + p.iqz;
-            PresDev &Q = a.dpres[dp++];
-            Q.c = p.preFactor * (float)(-2 * (((iu - m) / 2) % 2) + 1);
-            Q.q2n = p.q2n;
-            Q.a = p.iqx;
-            Q.b = p.iqy;
-            Q.cz = p.iqz;
-            Q.invq = p.invq;
-        }
-    }
-    a.key0 = (uint32_t)F.seed;
-    a.key1 = (uint32_t)(F.seed >> 32);
-    a.stream = (uint32_t)P.phi;
     if (!is_elementwise(a)) {
         if (int r = alloc_spec(&s->ft[0], g)) return r;
         if (int r = alloc_spec(&s->ft[1], g)) return r;
@@ -263,6 +321,14 @@ static void run_view(cupss_system s, bool elementwise, FusedInst &a)
         a.nat = F.spec[F.cur];
     }
     a.step_base = s->step;
+    a.cont = s->cont_now ? 1 : 0;
+    const int b0 = s->cont_now ? s->cont_buf : 0;
+    for (int c = 0; c < a.ncons; c++) {
+        FieldRec &C = s->f[s->plan.cons[c]];
+        a.cnat[c] = C.spec[C.cur];
+        a.cs[c][0] = s->fc[c][b0];
+        a.cs[c][1] = s->fc[c][b0 ^ 1];
+    }
     for (int k = 0; k < a.nreal; k++) a.zero[k] = (s->plan.real[k] == s->plan.phi && F.dstate == 0) ? 1 : 0;
     for (int d = 0; d < a.nder; d++) {
         FieldRec &D = s->f[s->plan.derived[d]];
@@ -290,7 +356,7 @@ static int upload_blocks(const std::vector<FusedBlock> &items, FusedBlock **dst,
 // bytes of the pipeline buffers one transform instance touches per step
 static double inst_bytes(const FusedInst &a)
 {
-    const int arrays = 2 + a.nreal + a.ngroups + (a.noise_mode == NOISE_REAL ? 1 : 0);
+    const int arrays = 2 + 2 * a.ncons + a.nreal + a.ngroups + (a.noise_mode == NOISE_REAL ? 1 : 0);
     return 8.0 * (double)a.g.nh * arrays;
 }
 
@@ -327,7 +393,7 @@ static int class_tables(const FusedBatch &B, const FusedBatch::Class &c, int lpb
 // 256 threads per column workgroup
 static void default_shapes(FusedBatch::Class &c)
 {
-    const std::vector<int> rs = fused_row_shapes(c.nc, c.ny), cs = fused_col_shapes(c.ny);
+    const std::vector<int> rs = fused_row_shapes(c.nc, c.ny, c.wide), cs = fused_col_shapes(c.ny);
     c.lpb = rs.empty() ? 0 : rs.front();
     for (int v : rs)
         if (v <= 8) c.lpb = v;
@@ -366,12 +432,13 @@ static int build(FusedBatch &B)
             for (long h0 = 0; h0 < a.g.nh / 2; h0 += FUSED_ELEM_BLOCK) er.push_back(FusedBlock{(int)i, (int)h0});
             continue;
         }
-        auto key = std::make_pair(a.g.nc, a.g.sy);
+        auto key = std::make_pair(a.g.nc * 2 + (a.nreal > 3 ? 1 : 0), a.g.sy);
         if (!cls.count(key)) {
             cls[key] = (int)B.classes.size();
             B.classes.emplace_back();
             B.classes.back().nc = a.g.nc;
             B.classes.back().ny = a.g.sy;
+            B.classes.back().wide = a.nreal > 3;
         }
         FusedBatch::Class &c = B.classes[cls[key]];
         c.insts.push_back((int)i);
@@ -428,7 +495,7 @@ static int launch_steps(FusedBatch &B, int n0, int k, int rel0)
     for (auto &c : B.classes)
         for (auto &t : c.chunks)
             for (int n = n0; n < n0 + k; n++) {
-                HIP_TRY(fused_rows_launch(c.nc, c.lpb, B.d, t.rb, t.nrb, (uint32_t)(n - rel0), B.stream));
+                HIP_TRY(fused_rows_launch(c.nc, c.lpb, c.wide, B.d, t.rb, t.nrb, (uint32_t)(n - rel0), B.stream));
                 HIP_TRY(fused_cols_launch(c.ny, c.twd, B.d, t.cb, t.ncb, n & 1, (uint32_t)(n - rel0), 1, B.stream));
             }
     return CUPSS_OK;
@@ -474,14 +541,14 @@ static int tune(FusedBatch &B)
         // rows
         float best = 1e30f;
         int pick = c.lpb;
-        for (int lpb : fused_row_shapes(c.nc, c.ny)) {
+        for (int lpb : fused_row_shapes(c.nc, c.ny, c.wide)) {
             std::vector<Tables> t;
             if (int r = class_tables(B, c, lpb, c.twd, 0, t)) return r;
             float ms = 0;
             const int r = time_ms(
                 B, T, reps,
                 [&]() -> int {
-                    HIP_TRY(fused_rows_launch(c.nc, lpb, B.d, t[0].rb, t[0].nrb, 0, B.stream));
+                    HIP_TRY(fused_rows_launch(c.nc, lpb, c.wide, B.d, t[0].rb, t[0].nrb, 0, B.stream));
                     return CUPSS_OK;
                 },
                 &ms);
@@ -569,6 +636,14 @@ static int settle(FusedBatch &B, int nsteps)
         if (F.order > 0) F.dstate = 2;
         for (int dfi : P.derived)
             if (s->f[dfi].order > 0) s->f[dfi].dstate = 2;
+        for (int cfi : P.cons)
+            if (s->f[cfi].order > 0) s->f[cfi].dstate = 2;
+        if (!P.cons.empty()) {
+            // the last pass left the next step's phase A/B in buffer nsteps&1
+            s->cont_buf = (s->cont_now ? s->cont_buf : 0) ^ (nsteps & 1);
+            s->cont_step = s->step;
+            s->cont_gen = s->gen;
+        }
         if (s->stream != B.stream) HIP_TRY(hipStreamWaitEvent(s->stream, B.done, 0));
     }
     return CUPSS_OK;
@@ -581,6 +656,24 @@ static int run(FusedBatch &B, int nsteps)
         if (s->stream != B.stream) HIP_TRY(hipStreamSynchronize(s->stream));
     if (stale_tables(B))
         if (int r = build(B)) return r;
+    // Systems with stored constraint fields: phase A/B of the run's first
+    // step with the general program (it reads the dealiased copies as left
+    // by earlier steps, whatever path made them); each pass then takes phi
+    // one step and the stored fields to the next step's phase A/B.
+    // A run continues from the previous fused run's values when nothing
+    // changed the system since (and no autotuning, which reuses the buffers).
+    const bool will_tune = B.autotune && !B.tuned && !B.classes.empty();
+    bool general = false;
+    for (cupss_system s : B.sys) {
+        s->cont_now = !s->plan.cons.empty() && !will_tune && s->cont_step == s->step && s->cont_gen == s->gen;
+        if (!s->plan.cons.empty() && !s->cont_now) {
+            if (int r = general_constraint_phase(s)) return r;
+            general = true;
+        }
+    }
+    if (general)
+        for (cupss_system s : B.sys)
+            if (s->stream != B.stream) HIP_TRY(hipStreamSynchronize(s->stream));
     if (int r = refresh(B)) return r;
     if (!twiddle_sets(B.stream)) return fail(CUPSS_ERR_NOMEM, "twiddle tables");
     if (int r = launch_prologue(B)) return r;
@@ -659,8 +752,8 @@ static std::string schedule_text(const FusedBatch &B)
     std::string out;
     char line[256];
     for (const auto &c : B.classes) {
-        std::snprintf(line, sizeof line, "class %dx%d: %zu instances, rows/wg %d, cols/wg %d, chunks %zu\n",
-                      2 * c.nc, c.ny, c.insts.size(), c.lpb, c.twd, c.chunks.size());
+        std::snprintf(line, sizeof line, "class %dx%d%s: %zu instances, rows/wg %d, cols/wg %d, chunks %zu\n",
+                      2 * c.nc, c.ny, c.wide ? " (wide)" : "", c.insts.size(), c.lpb, c.twd, c.chunks.size());
         out += line;
     }
     std::snprintf(line, sizeof line, "elementwise: %zu instances\nchunk target: %.0f MB%s\n", B.elem.size(),

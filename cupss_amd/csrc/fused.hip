@@ -41,9 +41,11 @@ namespace cupss {
 
 // ------------------------------------------------------------ row kernel
 
-template <int NC, int LPB>
+// MAXR: real fields a row holds at once (3, or FUSED_MAX_REAL for systems
+// with more; those use 4 elements per lane to stay within the registers)
+template <int NC, int LPB, int MAXR = 3>
 struct FRowCfg {
-    static constexpr int E = NC >= 1024 ? 16 : (NC >= 64 ? 8 : (NC >= 4 ? 4 : NC));
+    static constexpr int E = MAXR > 3 ? (NC >= 4 ? 4 : NC) : (NC >= 1024 ? 16 : (NC >= 64 ? 8 : (NC >= 4 ? 4 : NC)));
     static constexpr int T = NC / E;
     static constexpr int THREADS = LPB * T;
     static constexpr int RS = lds_len(NC);  // tile row stride (>= NC + 1), odd for NC >= 32
@@ -135,13 +137,13 @@ __device__ __forceinline__ void r2c_tile(float2 (&v)[E], float2 *L, int t, const
     __syncthreads();
 }
 
-template <int NC, int LPB>
-__global__ __launch_bounds__((FRowCfg<NC, LPB>::THREADS)) void k_frows(const FusedInst *__restrict__ insts,
-                                                                    const FusedBlock *__restrict__ blocks,
-                                                                    uint32_t step_rel,
-                                                                    const float2 *__restrict__ sets)
+template <int NC, int LPB, int MAXR>
+__global__ __launch_bounds__((FRowCfg<NC, LPB, MAXR>::THREADS)) void k_frows(const FusedInst *__restrict__ insts,
+                                                                          const FusedBlock *__restrict__ blocks,
+                                                                          uint32_t step_rel,
+                                                                          const float2 *__restrict__ sets)
 {
-    using C = FRowCfg<NC, LPB>;
+    using C = FRowCfg<NC, LPB, MAXR>;
     constexpr int E = C::E, T = C::T, RS = C::RS, TH = C::THREADS;
     static_assert(T % 2 == 0 && E % 2 == 0, "noise lanes are paired");
     __shared__ float2 tws[2 * NC];
@@ -155,9 +157,9 @@ __global__ __launch_bounds__((FRowCfg<NC, LPB>::THREADS)) void k_frows(const Fus
     const int nreal = a.nreal;
     load_twiddles<2 * NC>(tws, sets);
 
-    float2 r[FUSED_MAX_REAL][E];
+    float2 r[MAXR][E];
 #pragma unroll
-    for (int f = 0; f < FUSED_MAX_REAL; f++) {
+    for (int f = 0; f < MAXR; f++) {
         if (f < nreal) {
             tile_load<NC, LPB, RS, TH>(tile, a.Z[f], sy, y0, a.kmax[f]);
             __syncthreads();
@@ -181,14 +183,15 @@ __global__ __launch_bounds__((FRowCfg<NC, LPB>::THREADS)) void k_frows(const Fus
         float2 v[E];
 #pragma unroll
         for (int e = 0; e < E; e++) v[e] = make_float2(0.f, 0.f);
-        for (int pi = a.g0[gi]; pi < a.g0[gi + 1]; pi++) {
+        for (int pi = a.pg0[gi]; pi < a.pg0[gi + 1]; pi++) {
             float2 p[E];
+            const float c0 = a.coef[pi];
 #pragma unroll
-            for (int e = 0; e < E; e++) p[e] = make_float2(1.f, 1.f);
-            // product = prod_f r_f^cnt_f: static field index, uniform powers
-            // (no dynamic indexing, so r stays in registers)
+            for (int e = 0; e < E; e++) p[e] = make_float2(c0, c0);
+            // product = coef * prod_f r_f^cnt_f: static field index, uniform
+            // powers (no dynamic indexing, so r stays in registers)
 #pragma unroll
-            for (int f = 0; f < FUSED_MAX_REAL; f++)
+            for (int f = 0; f < MAXR; f++)
                 for (int c = 0; c < a.cnt[pi][f]; c++) {
 #pragma unroll
                     for (int e = 0; e < E; e++) {
@@ -231,6 +234,75 @@ __global__ __launch_bounds__((FRowCfg<NC, LPB>::THREADS)) void k_frows(const Fus
 
 // ------------------------------------------------------------ column kernel
 
+// Sum of the terms of one target field over the E bins of a lane: product
+// groups (FFT_y of X_g times the group multiplier) and single-field or
+// constant terms.  xv holds X of the first group on entry when `pre`.
+template <int N, int E, int T>
+__device__ __forceinline__ void target_sum(const FusedInst &a, const FusedTarget &TG, const ColBins<E> &cb,
+                                           const float2 (&old)[E], const float2 (&nw)[E], float2 (&xv)[E],
+                                           bool pre, int parity, bool active, long col, float2 *L, int t,
+                                           const float2 *tws, float2 (&acc)[E])
+{
+    float m[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) acc[e] = make_float2(0.f, 0.f);
+    for (int gi = TG.g0; gi < TG.g1; gi++) {
+        if (!(pre && gi == TG.g0)) {
+#pragma unroll
+            for (int e = 0; e < E; e++) xv[e] = active ? a.X[gi][col + t + e * T] : make_float2(0.f, 0.f);
+        }
+        fft_line<N, E, -1, N>(xv, L, t, tws);
+        const TermRefDev &GM = a.gmul[gi];
+        multiplier_col<E>(a.pool, GM.pres0, GM.npres, GM.mult_i, cb, m);
+#pragma unroll
+        for (int e = 0; e < E; e++) acc[e] = cadd(acc[e], apply_mult(m[e], GM.mult_i, xv[e]));
+        __syncthreads();
+    }
+    for (int ti = 0; ti < TG.nterms; ti++) {
+        const FusedTerm &TR = TG.term[ti];
+        multiplier_col<E>(a.pool, TR.pres0, TR.npres, TR.mult_i, cb, m);
+        float2 X[E];
+        if (TR.src == SRC_OLD || TR.src == SRC_NEW) {
+#pragma unroll
+            for (int e = 0; e < E; e++) X[e] = TR.src == SRC_OLD ? old[e] : nw[e];
+        } else if (TR.src == SRC_CONST) {
+#pragma unroll
+            for (int e = 0; e < E; e++) X[e] = cb.zero[e] ? make_float2((float)a.g.n, 0.f) : make_float2(0.f, 0.f);
+        } else if (TR.src == SRC_DERIVED) {
+            const TermRefDev &DM = a.dmul[TR.idx];
+            float dm[E];
+            multiplier_col<E>(a.pool, DM.pres0, DM.npres, DM.mult_i, cb, dm);
+#pragma unroll
+            for (int e = 0; e < E; e++) X[e] = apply_mult(dm[e], DM.mult_i, old[e]);
+        } else {
+            const float2 *__restrict__ src = a.cs[TR.idx][parity];
+#pragma unroll
+            for (int e = 0; e < E; e++) X[e] = active ? src[col + t + e * T] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int e = 0; e < E; e++) acc[e] = cadd(acc[e], apply_mult(m[e], TR.mult_i, X[e]));
+    }
+}
+
+// Z_f = IFFT_y(mask_f * val) for the columns the mask keeps
+template <int N, int E, int T>
+__device__ __forceinline__ void z_out(const FusedInst &a, int f, const Geom &g, const ColBins<E> &cb,
+                                      const float2 (&val)[E], bool zero, int line, bool active, long col, float2 *L,
+                                      int t, const float2 *tws)
+{
+    if (line > a.kmax[f]) return;  // uniform: every column of the workgroup is masked
+    float2 v[E];
+#pragma unroll
+    for (int e = 0; e < E; e++)
+        v[e] = (zero || !keep_mode2(g, a.kmax[f], a.kymax[f], cb.kx, t + e * T)) ? make_float2(0.f, 0.f) : val[e];
+    fft_line<N, E, +1, N>(v, L, t, tws);
+    if (active && cb.kx <= a.kmax[f]) {
+#pragma unroll
+        for (int e = 0; e < E; e++) a.Z[f][col + t + e * T] = v[e];
+    }
+    __syncthreads();
+}
+
 template <int N, int TWD>
 struct FColCfg {
     static constexpr int E = N >= 4096 ? 16 : (N >= 8 ? 8 : N);
@@ -260,7 +332,7 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
     float2 *L = lds + c * C::LL;
     const long col = (long)kx * N;
     const bool upd = mode == 1;
-    const int ngroups = upd ? a.ngroups : 0;
+    const bool pre = upd && a.tgt[0].g1 > a.tgt[0].g0;  // phi has product groups: load the first early
     const float2 *__restrict__ sin = a.spec[parity];
 
     float2 cur[E], xv[E], eta[E];
@@ -268,7 +340,7 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
     for (int e = 0; e < E; e++) {
         const long i = col + t + e * T;
         cur[e] = active ? sin[i] : make_float2(0.f, 0.f);
-        xv[e] = (active && ngroups > 0) ? a.X[0][i] : make_float2(0.f, 0.f);
+        xv[e] = (active && pre) ? a.X[a.tgt[0].g0][i] : make_float2(0.f, 0.f);
         eta[e] = (active && upd && a.noise_mode == NOISE_REAL) ? a.eta[i] : make_float2(0.f, 0.f);
     }
     load_twiddles<N>(tws, sets);
@@ -276,25 +348,12 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
 
     ColBins<E> cb;
     col_bins<E, T>(g, kx, t, cb);
+    float2 nw[E];  // phi_{m+1}
     if (upd) {
-        // phi is the dynamic field (setDynamic_k, src/field_kernels.cu:199-227)
         const uint64_t step = a.step_base + step_rel;
+        // ---- phi (setDynamic_k, src/field_kernels.cu:199-227)
         float2 acc[E];
-        float m[E];
-#pragma unroll
-        for (int e = 0; e < E; e++) acc[e] = make_float2(0.f, 0.f);
-        for (int gi = 0; gi < ngroups; gi++) {
-            if (gi > 0) {
-#pragma unroll
-                for (int e = 0; e < E; e++) xv[e] = active ? a.X[gi][col + t + e * T] : make_float2(0.f, 0.f);
-            }
-            fft_line<N, E, -1, N>(xv, L, t, tws);
-            const TermRefDev &TR = a.u.terms[a.gterm[gi]];
-            multiplier_col<E>(a.u.pres, TR.pres0, TR.npres, TR.mult_i, cb, m);
-#pragma unroll
-            for (int e = 0; e < E; e++) acc[e] = cadd(acc[e], apply_mult(m[e], TR.mult_i, xv[e]));
-            __syncthreads();
-        }
+        target_sum<N, E, T>(a, a.tgt[0], cb, cur, cur, xv, pre, parity, active, col, L, t, tws, acc);
         if (a.noise_mode == NOISE_REAL) {
             fft_line<N, E, -1, N>(eta, L, t, tws);
             __syncthreads();
@@ -304,19 +363,6 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
             for (int e = 0; e < E / 2; e++)
                 spectral_noise_pair(g, kx, t + e * T, 0, (uint32_t)step, (uint32_t)(step >> 32), a.stream, a.key0,
                                     a.key1, eta[e], eta[e + E / 2]);
-        }
-        // the remaining terms are constants or linear in phi itself
-        for (int ti = 0; ti < a.u.nterms; ti++) {
-            const TermRefDev &TR = a.u.terms[ti];
-            if (TR.kind == TERM_GROUPED) continue;
-            multiplier_col<E>(a.u.pres, TR.pres0, TR.npres, TR.mult_i, cb, m);
-#pragma unroll
-            for (int e = 0; e < E; e++) {
-                const float2 X = TR.kind == TERM_CONST
-                                     ? (cb.zero[e] ? make_float2((float)g.n, 0.f) : make_float2(0.f, 0.f))
-                                     : cur[e];
-                acc[e] = cadd(acc[e], apply_mult(m[e], TR.mult_i, X));
-            }
         }
         float rD[E];
         if (a.u.nimp > 0) implicit_col<E>(a.u, cb, rD);
@@ -334,40 +380,84 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
                 s.x *= rD[e];
                 s.y *= rD[e];
             }
-            cur[e] = s;
+            nw[e] = s;
         }
         if (active) {
             float2 *__restrict__ sout = a.spec[parity ^ 1];
 #pragma unroll
-            for (int e = 0; e < E; e++) sout[col + t + e * T] = cur[e];
+            for (int e = 0; e < E; e++) sout[col + t + e * T] = nw[e];
+        }
+        // ---- stored constraints (setNotDynamic_k, src/field_kernels.cu:130-197):
+        // their values of step m+1, from the pre-step state of step m+1
+        for (int ci = 0; ci < a.ncons; ci++) {
+            const FusedTarget &TG = a.tgt[1 + ci];
+            float2 v[E];
+            target_sum<N, E, T>(a, TG, cb, cur, nw, xv, false, parity, active, col, L, t, tws, v);
+            if (TG.nimp > 0) {
+                float lsum[E];
+#pragma unroll
+                for (int e = 0; e < E; e++) lsum[e] = 0.0f;
+                for (int i = 0; i < TG.nimp; i++) {
+                    const PresDev P = a.pool[TG.imp0 + i];
+                    float th[E];
+#pragma unroll
+                    for (int e = 0; e < E; e++) th[e] = P.c;
+                    for (int k = 0; k < P.q2n; k++)
+#pragma unroll
+                        for (int e = 0; e < E; e++) th[e] *= cb.q2[e];
+                    if (P.invq != 0) {
+#pragma unroll
+                        for (int e = 0; e < E; e++) {
+                            const float iq = col_iq(cb, e);
+                            for (int k = 0; k < P.invq; k++) th[e] *= iq;
+                        }
+                    }
+#pragma unroll
+                    for (int e = 0; e < E; e++) lsum[e] += th[e];
+                }
+#pragma unroll
+                for (int e = 0; e < E; e++)
+                    if (!cb.zero[e]) {
+                        const float rl = __builtin_amdgcn_rcpf(lsum[e]);
+                        v[e].x *= rl;
+                        v[e].y *= rl;
+                    }
+            }
+            if (active) {
+                float2 *__restrict__ cout = a.cs[ci][parity ^ 1];
+#pragma unroll
+                for (int e = 0; e < E; e++) cout[col + t + e * T] = v[e];
+            }
+            if (TG.z >= 0) z_out<N, E, T>(a, TG.z, g, cb, v, false, blk.line, active, col, L, t, tws);
+        }
+    } else {
+        // prologue: the real fields of the current state
+#pragma unroll
+        for (int e = 0; e < E; e++) nw[e] = cur[e];
+        for (int ci = 0; ci < a.ncons; ci++) {
+            const int z = a.tgt[1 + ci].z;
+            if (z < 0) continue;
+            float2 v[E];
+#pragma unroll
+            for (int e = 0; e < E; e++) v[e] = active ? a.cs[ci][0][col + t + e * T] : make_float2(0.f, 0.f);
+            z_out<N, E, T>(a, z, g, cb, v, a.zero[z] != 0, blk.line, active, col, L, t, tws);
         }
     }
-#pragma unroll
-    for (int f = 0; f < FUSED_MAX_REAL; f++) {
-        // workgroups whose columns the mask removes entirely skip the transform
-        if (f < a.nreal && blk.line <= a.kmax[f]) {
-            float2 v[E];
-            const bool zero = !upd && a.zero[f];
+    // real fields that are phi or derived from it: L_f phi_{m+1}
+    for (int f = 0; f < a.nreal; f++) {
+        if (a.zsrc[f] >= 0) continue;
+        float2 v[E];
+        const TermRefDev &LM = a.lmul[f];
+        if (LM.npres > 0) {
             float lm[E];
-            const TermRefDev &LM = a.lmul[f];
-            if (LM.npres > 0) multiplier_col<E>(a.lpres, LM.pres0, LM.npres, LM.mult_i, cb, lm);
+            multiplier_col<E>(a.pool, LM.pres0, LM.npres, LM.mult_i, cb, lm);
 #pragma unroll
-            for (int e = 0; e < E; e++) {
-                const int ky = t + e * T;
-                float2 w = cur[e];
-                if (zero || !keep_mode2(g, a.kmax[f], a.kymax[f], kx, ky))
-                    w = make_float2(0.f, 0.f);
-                else if (LM.npres > 0)
-                    w = apply_mult(lm[e], LM.mult_i, w);
-                v[e] = w;
-            }
-            fft_line<N, E, +1, N>(v, L, t, tws);
-            if (active && kx <= a.kmax[f]) {
+            for (int e = 0; e < E; e++) v[e] = apply_mult(lm[e], LM.mult_i, nw[e]);
+        } else {
 #pragma unroll
-                for (int e = 0; e < E; e++) a.Z[f][col + t + e * T] = v[e];
-            }
-            __syncthreads();
+            for (int e = 0; e < E; e++) v[e] = nw[e];
         }
+        z_out<N, E, T>(a, f, g, cb, v, !upd && a.zero[f] != 0, blk.line, active, col, L, t, tws);
     }
 }
 
@@ -382,67 +472,82 @@ __device__ __forceinline__ void tile_origin(const Geom &g, int line, int &x0, in
     y0 = (line / ntx) * FUSED_TILE;
 }
 
-__global__ __launch_bounds__(256) void k_ftrans_in(const FusedInst *__restrict__ insts,
-                                                   const FusedBlock *__restrict__ blocks)
+// one tile of a natural half spectrum into a column-major one
+__device__ __forceinline__ void tile_to_cm(float2 (&s)[FUSED_TILE][FUSED_TILE + 1], const Geom &g, int x0, int y0,
+                                           const float2 *__restrict__ src, float2 *__restrict__ dst)
 {
     constexpr int S = FUSED_TILE;
-    __shared__ float2 s[S][S + 1];
-    const FusedBlock blk = blocks[blockIdx.x];
-    if (blk.inst < 0) return;
-    const FusedInst &a = insts[blk.inst];
-    const Geom &g = a.g;
-    int x0, y0;
-    tile_origin(g, blk.line, x0, y0);
     for (int q = threadIdx.x; q < S * S; q += blockDim.x) {
         const int i = q % S, j = q / S;
-        if (x0 + i < g.nc1 && y0 + j < g.sy) s[j][i] = a.nat[(long)(y0 + j) * g.nc1 + x0 + i];
+        if (x0 + i < g.nc1 && y0 + j < g.sy) s[j][i] = src[(long)(y0 + j) * g.nc1 + x0 + i];
     }
     __syncthreads();
     for (int q = threadIdx.x; q < S * S; q += blockDim.x) {
         const int j = q % S, i = q / S;
-        if (x0 + i < g.nc1 && y0 + j < g.sy) a.spec[0][(long)(x0 + i) * g.sy + y0 + j] = s[j][i];
+        if (x0 + i < g.nc1 && y0 + j < g.sy) dst[(long)(x0 + i) * g.sy + y0 + j] = s[j][i];
     }
+    __syncthreads();
 }
 
-// column-major spec[fin] -> natural spectrum; the derived fields of the
-// last step's starting phi (spec[fin^1]) in natural layout
-__global__ __launch_bounds__(256) void k_fsettle(const FusedInst *__restrict__ insts,
-                                                 const FusedBlock *__restrict__ blocks, int fin)
+// one tile of a column-major half spectrum into a natural one, each value
+// times M (npres = 0: copied)
+__device__ __forceinline__ void tile_to_nat(float2 (&s)[FUSED_TILE][FUSED_TILE + 1], const FusedInst &a, int x0,
+                                            int y0, const float2 *__restrict__ src, float2 *__restrict__ dst,
+                                            const TermRefDev &M)
 {
     constexpr int S = FUSED_TILE;
-    __shared__ float2 s[S][S + 1];
+    const Geom &g = a.g;
+    for (int q = threadIdx.x; q < S * S; q += blockDim.x) {
+        const int j = q % S, i = q / S;
+        if (x0 + i < g.nc1 && y0 + j < g.sy) s[j][i] = src[(long)(x0 + i) * g.sy + y0 + j];
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < S * S; q += blockDim.x) {
+        const int i = q % S, j = q / S;
+        const int kx = x0 + i, ky = y0 + j;
+        if (kx < g.nc1 && ky < g.sy) {
+            const long h = (long)ky * g.nc1 + kx;
+            if (M.npres == 0) {
+                dst[h] = s[j][i];
+            } else {
+                const Bin b = make_bin(g, kx, ky, 0);
+                dst[h] = apply_mult(multiplier(a.pool, M.pres0, M.npres, M.mult_i, b), M.mult_i, s[j][i]);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_ftrans_in(const FusedInst *__restrict__ insts,
+                                                   const FusedBlock *__restrict__ blocks)
+{
+    __shared__ float2 s[FUSED_TILE][FUSED_TILE + 1];
     const FusedBlock blk = blocks[blockIdx.x];
     if (blk.inst < 0) return;
     const FusedInst &a = insts[blk.inst];
-    const Geom &g = a.g;
     int x0, y0;
-    tile_origin(g, blk.line, x0, y0);
-    for (int pass = 0; pass < (a.nder > 0 ? 2 : 1); pass++) {
-        const float2 *__restrict__ src = a.spec[pass == 0 ? fin : fin ^ 1];
-        for (int q = threadIdx.x; q < S * S; q += blockDim.x) {
-            const int j = q % S, i = q / S;
-            if (x0 + i < g.nc1 && y0 + j < g.sy) s[j][i] = src[(long)(x0 + i) * g.sy + y0 + j];
-        }
-        __syncthreads();
-        for (int q = threadIdx.x; q < S * S; q += blockDim.x) {
-            const int i = q % S, j = q / S;
-            const int kx = x0 + i, ky = y0 + j;
-            if (kx < g.nc1 && ky < g.sy) {
-                const long h = (long)ky * g.nc1 + kx;
-                if (pass == 0) {
-                    a.nat[h] = s[j][i];
-                } else {
-                    const Bin b = make_bin(g, kx, ky, 0);
-                    for (int d = 0; d < a.nder; d++) {
-                        const TermRefDev &M = a.dmul[d];
-                        a.dout[d][h] = apply_mult(multiplier(a.dpres, M.pres0, M.npres, M.mult_i, b), M.mult_i,
-                                                  s[j][i]);
-                    }
-                }
-            }
-        }
-        __syncthreads();
-    }
+    tile_origin(a.g, blk.line, x0, y0);
+    tile_to_cm(s, a.g, x0, y0, a.nat, a.spec[0]);
+    if (!a.cont)
+        for (int c = 0; c < a.ncons; c++) tile_to_cm(s, a.g, x0, y0, a.cnat[c], a.cs[c][0]);
+}
+
+// column-major phi [fin] -> natural; stored constraints of the last step's
+// start ([fin^1]) -> natural; the derived fields of the last step's
+// starting phi ([fin^1]) in natural layout
+__global__ __launch_bounds__(256) void k_fsettle(const FusedInst *__restrict__ insts,
+                                                 const FusedBlock *__restrict__ blocks, int fin)
+{
+    __shared__ float2 s[FUSED_TILE][FUSED_TILE + 1];
+    const FusedBlock blk = blocks[blockIdx.x];
+    if (blk.inst < 0) return;
+    const FusedInst &a = insts[blk.inst];
+    int x0, y0;
+    tile_origin(a.g, blk.line, x0, y0);
+    const TermRefDev copy{nullptr, 0, 0, 0, 0};
+    tile_to_nat(s, a, x0, y0, a.spec[fin], a.nat, copy);
+    for (int c = 0; c < a.ncons; c++) tile_to_nat(s, a, x0, y0, a.cs[c][fin ^ 1], a.cnat[c], copy);
+    for (int d = 0; d < a.nder; d++) tile_to_nat(s, a, x0, y0, a.spec[fin ^ 1], a.dout[d], a.dmul[d]);
 }
 
 // ------------------------------------------------------------ elementwise
@@ -509,7 +614,7 @@ __global__ __launch_bounds__(FUSED_ELEM_BLOCK) void k_felem(const FusedInst *__r
 #pragma unroll
                 for (int w = 0; w < 2; w++)
                     a.dout[d][b[w].h] =
-                        apply_mult(multiplier(a.dpres, M.pres0, M.npres, M.mult_i, b[w]), M.mult_i, cur[w]);
+                        apply_mult(multiplier(a.pool, M.pres0, M.npres, M.mult_i, b[w]), M.mult_i, cur[w]);
             }
         }
         float2 eta[2] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
@@ -548,23 +653,27 @@ __global__ __launch_bounds__(FUSED_ELEM_BLOCK) void k_felem(const FusedInst *__r
 
 bool fused_supported(int sx, int sy, int sz) { return sz == 1 && sx >= 16 && sx <= 4096 && sy >= 8 && sy <= 4096; }
 
-template <int NC>
+// wide row kernels (more than 3 real fields) for the sizes such systems use
+#define FUSED_WIDE_NC(NC) ((NC) >= 32 && (NC) <= 1024)
+
+template <int NC, int MAXR>
 static std::vector<int> row_shapes(int ny)
 {
     std::vector<int> v;
 #define X(LPB) \
-    if (FRowCfg<NC, LPB>::valid() && LPB <= ny) v.push_back(LPB);
+    if (FRowCfg<NC, LPB, MAXR>::valid() && LPB <= ny) v.push_back(LPB);
     FUSED_LPB(X)
 #undef X
     return v;
 }
 
-std::vector<int> fused_row_shapes(int nc, int ny)
+std::vector<int> fused_row_shapes(int nc, int ny, bool wide)
 {
     switch (nc) {
-#define X(NC) \
-    case NC:  \
-        return row_shapes<NC>(ny);
+#define X(NC)                                                                                       \
+    case NC:                                                                                        \
+        if (wide) return FUSED_WIDE_NC(NC) ? row_shapes<NC, FUSED_MAX_REAL>(ny) : std::vector<int>(); \
+        return row_shapes<NC, 3>(ny);
         FUSED_ROW_CASES(X)
 #undef X
     default:
@@ -596,32 +705,37 @@ std::vector<int> fused_col_shapes(int ny)
     }
 }
 
-template <int NC>
+template <int NC, int MAXR>
 static hipError_t frows_n(int lpb, const FusedInst *inst, const FusedBlock *blocks, int nblocks, uint32_t step_rel,
                           hipStream_t s, const float2 *sets)
 {
-#define X(LPB)                                                                                                  \
-    if constexpr (FRowCfg<NC, LPB>::valid()) {                                                                 \
-        if (lpb == LPB) {                                                                                       \
-            hipLaunchKernelGGL((k_frows<NC, LPB>), dim3(nblocks), dim3(FRowCfg<NC, LPB>::THREADS), 0, s, inst, \
-                               blocks, step_rel, sets);                                                         \
-            return hipGetLastError();                                                                           \
-        }                                                                                                       \
+#define X(LPB)                                                                                                   \
+    if constexpr (FRowCfg<NC, LPB, MAXR>::valid()) {                                                            \
+        if (lpb == LPB) {                                                                                        \
+            hipLaunchKernelGGL((k_frows<NC, LPB, MAXR>), dim3(nblocks), dim3(FRowCfg<NC, LPB, MAXR>::THREADS), 0, \
+                               s, inst, blocks, step_rel, sets);                                                 \
+            return hipGetLastError();                                                                            \
+        }                                                                                                        \
     }
     FUSED_LPB(X)
 #undef X
     return hipErrorInvalidValue;
 }
 
-hipError_t fused_rows_launch(int nc, int lpb, const FusedInst *inst, const FusedBlock *blocks, int nblocks,
-                             uint32_t step_rel, hipStream_t s)
+hipError_t fused_rows_launch(int nc, int lpb, bool wide, const FusedInst *inst, const FusedBlock *blocks,
+                             int nblocks, uint32_t step_rel, hipStream_t s)
 {
     const float2 *sets = twiddle_sets(s);
     if (!sets) return hipErrorOutOfMemory;
     switch (nc) {
-#define X(NC) \
-    case NC:  \
-        return frows_n<NC>(lpb, inst, blocks, nblocks, step_rel, s, sets);
+#define X(NC)                                                                                 \
+    case NC:                                                                                  \
+        if (wide) {                                                                           \
+            if constexpr (FUSED_WIDE_NC(NC))                                                  \
+                return frows_n<NC, FUSED_MAX_REAL>(lpb, inst, blocks, nblocks, step_rel, s, sets); \
+            return hipErrorInvalidValue;                                                      \
+        }                                                                                     \
+        return frows_n<NC, 3>(lpb, inst, blocks, nblocks, step_rel, s, sets);
         FUSED_ROW_CASES(X)
 #undef X
     default:

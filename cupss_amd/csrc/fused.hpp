@@ -1,6 +1,14 @@
 // fused.hpp -- the fused two-kernel step for 2D systems with one dynamic
-// field phi whose other fields are linear in phi (fused.hip), batched over
-// any number of independent instances (batch.hip).
+// field phi (fused.hip), batched over any number of independent instances
+// (batch.hip).
+//
+// The other fields of such a system are
+//   derived:  psi = M_psi phi (one single-field term of phi, no implicit
+//             operator, no noise) -- never stored during a run, evaluated
+//             from phi where needed;
+//   stored:   any other constraint field without noise whose product terms
+//             read only derived or stored fields -- kept as state, updated
+//             every pass (the "phase A/B" of the next step, see fused.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -11,19 +19,45 @@
 
 namespace cupss {
 
-constexpr int FUSED_MAX_REAL = 3;    // dealiased real fields read by the products
-constexpr int FUSED_MAX_GROUPS = 4;  // groups of product terms with equal prefactors
-constexpr int FUSED_MAX_PRODS = 8;   // product terms over all groups
-constexpr int FUSED_MAX_FAC = 4;     // factors of one product
-constexpr int FUSED_MAX_LPRES = 16;  // prefactors of the derived-field multipliers
-constexpr int FUSED_MAX_DERIVED = 4; // derived fields written back after a run
-constexpr int FUSED_TILE = 32;       // side of the transpose tiles
-constexpr int FUSED_ELEM_BLOCK = 256;  // bin pairs per workgroup of the elementwise kernel
+constexpr int FUSED_MAX_REAL = 6;     // dealiased real fields read by the products
+constexpr int FUSED_MAX_GROUPS = 8;   // groups of product terms (one transform each)
+constexpr int FUSED_MAX_PRODS = 12;   // product terms over all groups
+constexpr int FUSED_MAX_CONS = 6;     // stored constraint fields
+constexpr int FUSED_MAX_TARGETS = 1 + FUSED_MAX_CONS;
+constexpr int FUSED_MAX_TTERMS = 8;   // non-product terms of one field
+constexpr int FUSED_MAX_POOL = 96;    // prefactors of all multipliers of an instance
+constexpr int FUSED_MAX_DERIVED = 4;  // derived fields written back after a run
+constexpr int FUSED_TILE = 32;        // side of the transpose tiles
+constexpr int FUSED_ELEM_BLOCK = 256; // bin pairs per workgroup of the elementwise kernel
 
 enum NoiseMode : int {
     NOISE_NONE = 0,
     NOISE_REAL = 1,      // xi ~ N(0,1) per real-space cell, then R2C (the reference's way)
     NOISE_SPECTRAL = 2,  // eta_hat drawn directly with the law of R2C(xi)
+};
+
+// Source of a non-product term (one factor, or none) in a pass that takes
+// phi from step m to m+1 and the stored constraints from m to m+1.
+enum FusedSrc : int {
+    SRC_OLD = 0,      // phi_m
+    SRC_NEW = 1,      // phi_{m+1} (terms of constraint fields: phi's pre-step value of step m+1)
+    SRC_CONST = 2,    // no factor: N at q = 0
+    SRC_DERIVED = 3,  // derived field idx: M_idx phi_m
+    SRC_CONS = 4,     // stored constraint idx at step m
+};
+
+struct FusedTerm {
+    int src, idx;
+    int mult_i, pres0, npres;  // multiplier in the pool
+};
+
+// One field a pass updates: phi (target 0) or stored constraint c (1 + c)
+struct FusedTarget {
+    int nterms;
+    FusedTerm term[FUSED_MAX_TTERMS];
+    int g0, g1;      // its product groups [g0, g1)
+    int nimp, imp0;  // constraint: implicit prefactors in the pool (division for q != 0)
+    int z;           // constraint: index of its real field, -1 if none
 };
 
 // Device-side descriptor of one instance (one system) of a fused batch.
@@ -36,32 +70,38 @@ struct FusedInst {
     Geom g;
     float2 *spec[2];                          // phi's spectrum: [parity] in, [parity^1] out
     float2 *nat;                              // the system's spectrum (natural layout), transform instances
-    float2 *Z[FUSED_MAX_REAL];                // IFFT_y(mask_f L_f phi_hat), columns kx <= kmax[f]
-    float2 *X[FUSED_MAX_GROUPS];              // R2C_x(products of group g)
+    float2 *cs[FUSED_MAX_CONS][2];            // stored constraints, column-major, like spec
+    float2 *cnat[FUSED_MAX_CONS];             // their system buffers (natural layout)
+    float2 *Z[FUSED_MAX_REAL];                // IFFT_y(mask_f * field_f), columns kx <= kmax[f]
+    float2 *X[FUSED_MAX_GROUPS];              // R2C_x(sum of the products of group g)
     float2 *eta;                              // R2C_x(xi) rows (NOISE_REAL)
     uint64_t step_base;                       // step counter of the first step of a launch sequence
     // row pass
     float scale;                              // 1/N of the inverse transform
     int nreal;
     int ngroups;
-    int g0[FUSED_MAX_GROUPS + 1];             // products of group g: [g0[g], g0[g+1])
+    int pg0[FUSED_MAX_GROUPS + 1];            // products of group g: [pg0[g], pg0[g+1])
     int cnt[FUSED_MAX_PRODS][FUSED_MAX_REAL]; // power of each real field in product p
+    float coef[FUSED_MAX_PRODS];              // real-space coefficient of product p
     int noise_mode;
     uint32_t key0, key1, stream;
     // column pass
-    UpdateArgs u;                             // phi's update; grouped product terms are TERM_GROUPED
-    int gterm[FUSED_MAX_GROUPS];              // u.terms index carrying group g's prefactors
+    UpdateArgs u;                             // phi: dt, noise, implicit D(q) (and its terms, elementwise kernel)
+    int ncons;
+    int cont;                                 // cs[.][0] already holds this run's first phase A/B
+    FusedTarget tgt[FUSED_MAX_TARGETS];
+    TermRefDev gmul[FUSED_MAX_GROUPS];        // multiplier of each group (pool)
+    PresDev pool[FUSED_MAX_POOL];
     int p[FUSED_MAX_REAL];                    // dealias order of each real field
     int kmax[FUSED_MAX_REAL];                 // last column kx the dealias mask of field f keeps
     int kymax[FUSED_MAX_REAL];                // largest |ky| the mask keeps
     int zero[FUSED_MAX_REAL];                 // real field is zero in the prologue (step-0 state)
-    TermRefDev lmul[FUSED_MAX_REAL];          // L_f (npres = 0: identity)
-    PresDev lpres[FUSED_MAX_LPRES];
+    int zsrc[FUSED_MAX_REAL];                 // -1: L_f phi (lmul); c >= 0: stored constraint c
+    TermRefDev lmul[FUSED_MAX_REAL];          // L_f (npres = 0: identity; pool)
     // derived fields psi_d = P(M_d phi_old), written after a run
     int nder;
     float2 *dout[FUSED_MAX_DERIVED];
-    TermRefDev dmul[FUSED_MAX_DERIVED];
-    PresDev dpres[FUSED_MAX_LPRES];
+    TermRefDev dmul[FUSED_MAX_DERIVED];       // pool
 };
 
 // Work item of one workgroup: instance and first row / column / tile / bin
@@ -74,23 +114,27 @@ struct FusedBlock {
 bool fused_supported(int sx, int sy, int sz);
 
 // Tile shapes a size class can be launched with: rows per workgroup of the
-// row kernel (nc = sx/2) and columns per workgroup of the column kernel
-// (ny = sy).  Any choice gives bit-identical results.
-std::vector<int> fused_row_shapes(int nc, int ny);
+// row kernel (nc = sx/2; wide = more than 3 real fields) and columns per
+// workgroup of the column kernel (ny = sy).  Any choice gives bit-identical
+// results.
+std::vector<int> fused_row_shapes(int nc, int ny, bool wide);
 std::vector<int> fused_col_shapes(int ny);
 
-hipError_t fused_rows_launch(int nc, int lpb, const FusedInst *inst, const FusedBlock *blocks, int nblocks,
-                             uint32_t step_rel, hipStream_t s);
+hipError_t fused_rows_launch(int nc, int lpb, bool wide, const FusedInst *inst, const FusedBlock *blocks,
+                             int nblocks, uint32_t step_rel, hipStream_t s);
 // mode 0: prologue (real fields of the current state), 1: step
 hipError_t fused_cols_launch(int ny, int twd, const FusedInst *inst, const FusedBlock *blocks, int nblocks,
                              int parity, uint32_t step_rel, int mode, hipStream_t s);
-// natural spectrum -> column-major spec[0] (blocks: one tile each)
+// natural spectra -> column-major buffer 0 (phi and the stored constraints;
+// blocks: one tile each)
 hipError_t fused_transpose_in_launch(const FusedInst *inst, const FusedBlock *blocks, int nblocks, hipStream_t s);
-// column-major spec[fin] -> natural spectrum, derived fields from spec[fin^1]
+// column-major phi [fin] and stored constraints [fin^1] -> natural, derived
+// fields from phi [fin^1]
 hipError_t fused_settle_launch(const FusedInst *inst, const FusedBlock *blocks, int nblocks, int fin,
                                hipStream_t s);
-// nsteps steps of instances with no products and no real-space noise
-// (blocks: FUSED_ELEM_BLOCK bin pairs each); derived fields from the last step's start
+// nsteps steps of instances with no products, no stored constraints and no
+// real-space noise (blocks: FUSED_ELEM_BLOCK bin pairs each); derived fields
+// from the last step's start
 hipError_t fused_elem_launch(const FusedInst *inst, const FusedBlock *blocks, int nblocks, int parity,
                              uint32_t step_rel, int nsteps, hipStream_t s);
 

@@ -119,6 +119,10 @@ extern "C" int cupss_system_destroy(cupss_system s)
     for (auto *p : s->fz) hipFree(p);
     for (auto *p : s->fx) hipFree(p);
     for (auto *p : s->ft) hipFree(p);
+    for (auto &c : s->fc) {
+        hipFree(c[0]);
+        hipFree(c[1]);
+    }
     hipFree(s->scratch);
     hipFree(s->rtmp);
     hipFree(s->full);
@@ -233,6 +237,7 @@ static void plan_fused(cupss_system s);
 extern "C" int cupss_field_set_real(cupss_system s, int field, const float *host)
 {
     if (int r = check_field(s, field)) return r;
+    s->cont_step = ~0ull;
     if (!host) return fail(CUPSS_ERR_INVALID, "null host pointer");
     // the dealiased copy keeps its old values (prepareProblem re-runs only
     // toComp, src/evolver.cpp:114-117): write them out before the spectrum
@@ -288,6 +293,7 @@ extern "C" int cupss_field_get_spectrum(cupss_system s, int field, float *host)
 extern "C" int cupss_field_set_spectrum(cupss_system s, int field, const float *host)
 {
     if (int r = check_field(s, field)) return r;
+    s->cont_step = ~0ull;
     FieldRec &F = s->f[field];
     HIP_TRY(hipMemcpyAsync(F.spec[F.cur], host, sizeof(float2) * s->g.nh, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -300,6 +306,7 @@ extern "C" int cupss_field_spectrum_device(cupss_system s, int field, void **dev
 {
     if (int r = check_field(s, field)) return r;
     *dev_ptr = s->f[field].spec[s->f[field].cur];
+    s->cont_step = ~0ull;  // the caller may write the spectrum
     return CUPSS_OK;
 }
 
@@ -576,8 +583,16 @@ static int run_phase(cupss_system s, int dynamic)
     return CUPSS_OK;
 }
 
+int cupss::general_constraint_phase(cupss_system s)
+{
+    for (int i = 0; i < (int)s->f.size(); i++)
+        if (int r = materialize(s, i)) return r;
+    return run_phase(s, 0);
+}
+
 int cupss::general_steps(cupss_system s, int nsteps)
 {
+    if (nsteps > 0) s->cont_step = ~0ull;
     for (int i = 0; i < (int)s->f.size(); i++)
         if (int r = materialize(s, i)) return r;
     for (int n = 0; n < nsteps; n++) {
@@ -702,7 +717,7 @@ extern "C" int cupss_system_device_bytes(cupss_system s, size_t *bytes)
     for (auto &T : s->t)
         if (T.xhat) b += spec;
     if (s->plan.ok)  // the fused path's buffers, allocated at its first run
-        b += spec * (2 + s->plan.real.size() + s->plan.groups.size());
+        b += spec * (2 + 2 * s->plan.cons.size() + s->plan.real.size() + s->plan.groups.size());
     *bytes = b;
     return CUPSS_OK;
 }
@@ -740,69 +755,141 @@ static bool same_pres(const std::vector<cupss_pres> &a, const std::vector<cupss_
     return true;
 }
 
-// Is the system one dynamic field phi plus derived fields psi = M_psi phi
-// (constraint, one single-field term of phi, no implicit, no noise) that
-// appear only in phi's products?  Then the fused 2D step applies.
+static bool single_monomial(const std::vector<cupss_pres> &a)
+{
+    return a.size() == 1;
+}
+
+static bool same_monomial(const cupss_pres &a, const cupss_pres &b)
+{
+    return a.q2n == b.q2n && a.iqx == b.iqx && a.iqy == b.iqy && a.iqz == b.iqz && a.invq == b.invq;
+}
+
+// Does the fused 2D step apply, and how?  One dynamic field phi; every other
+// field is either derived (psi = M_psi phi: one single-field term of phi,
+// no implicit operator, no noise) or stored (a constraint without noise
+// whose product terms read only derived or stored fields -- a product with
+// phi as a factor would need phi's dealiased copy one step early).  phi's
+// own terms may read any field.
 static void plan_fused(cupss_system s)
 {
     auto &P = s->plan;
     P = FusedPlan();
     if (!fused_supported(s->g.sx, s->g.sy, s->g.sz)) return;
+    const int nf = (int)s->f.size();
     int phi = -1;
-    for (int i = 0; i < (int)s->f.size(); i++)
+    for (int i = 0; i < nf; i++)
         if (s->f[i].dynamic) {
             if (phi >= 0) return;
             phi = i;
         }
     if (phi < 0) return;
-    std::vector<char> derived(s->f.size(), 0);
-    for (int i = 0; i < (int)s->f.size(); i++) {
+    std::vector<char> derived(nf, 0), stored(nf, 0);
+    for (int i = 0; i < nf; i++) {
         if (i == phi) continue;
         const FieldRec &F = s->f[i];
-        if (!F.imp.empty() || F.noisy || F.terms.size() != 1) return;
-        const TermRec &T = s->t[F.terms[0]];
-        if (T.fac.size() != 1 || T.fac[0] != phi) return;
-        if ((int)T.pres.size() > FUSED_MAX_LPRES) return;
-        derived[i] = 1;
-    }
-    // phi's terms
-    std::vector<int> prods;
-    for (int ti : s->f[phi].terms) {
-        const TermRec &T = s->t[ti];
-        if (T.fac.size() == 1) {
-            if (T.fac[0] != phi) return;   // linear terms of phi only
-        } else if (T.fac.size() >= 2) {
-            if ((int)T.fac.size() > FUSED_MAX_FAC) return;
-            for (int c : T.fac) {
-                if (c != phi && !derived[c]) return;
-                if (std::find(P.real.begin(), P.real.end(), c) == P.real.end()) P.real.push_back(c);
+        if (F.noisy) return;
+        if (F.imp.empty() && F.terms.size() == 1) {
+            const TermRec &T = s->t[F.terms[0]];
+            if (T.fac.size() == 1 && T.fac[0] == phi && (int)T.pres.size() <= FUSED_MAX_POOL / 8) {
+                derived[i] = 1;
+                continue;
             }
-            prods.push_back(ti);
         }
+        if (F.terms.empty()) return;  // keeps its value: the general program's case
+        stored[i] = 1;
     }
-    if ((int)P.real.size() > FUSED_MAX_REAL || (int)prods.size() > FUSED_MAX_PRODS) return;
-    int lp = 0;
-    for (int f : P.real)
-        if (f != phi) lp += (int)s->t[s->f[f].terms[0]].pres.size();
-    if (lp > FUSED_MAX_LPRES) return;
-    for (int ti : prods) {
-        bool placed = false;
-        for (auto &grp : P.groups)
-            if (same_pres(s->t[grp[0]].pres, s->t[ti].pres)) {
-                grp.push_back(ti);
-                placed = true;
-                break;
+    // product factors: phi's products read anything; stored fields' products
+    // read derived or stored fields only
+    std::vector<std::vector<int>> prods_of(nf);
+    for (int i = 0; i < nf; i++) {
+        if (derived[i]) continue;
+        int single = 0;
+        for (int ti : s->f[i].terms) {
+            const TermRec &T = s->t[ti];
+            if (T.fac.size() >= 2) {
+                if ((int)T.fac.size() > 8) return;
+                for (int c : T.fac) {
+                    if (i != phi && c == phi) return;
+                    if (std::find(P.real.begin(), P.real.end(), c) == P.real.end()) P.real.push_back(c);
+                }
+                prods_of[i].push_back(ti);
+            } else {
+                single++;
             }
-        if (!placed) P.groups.push_back({ti});
+        }
+        if (single > FUSED_MAX_TTERMS) return;
     }
-    if ((int)P.groups.size() > FUSED_MAX_GROUPS) return;
-    for (int i = 0; i < (int)s->f.size(); i++)
+    if ((int)P.real.size() > FUSED_MAX_REAL) return;
+    for (int i = 0; i < nf; i++)
+        if (stored[i]) P.cons.push_back(i);
+    if ((int)P.cons.size() > FUSED_MAX_CONS) return;
+    // groups: phi's first, then each stored field's
+    std::vector<int> order = {phi};
+    order.insert(order.end(), P.cons.begin(), P.cons.end());
+    int nprod = 0;
+    for (int f : order) {
+        std::vector<FusedGroup> gs;
+        for (int ti : prods_of[f]) {
+            const std::vector<cupss_pres> &pr = s->t[ti].pres;
+            bool placed = false;
+            for (auto &grp : gs)
+                if (same_pres(grp.pres, pr)) {
+                    grp.terms.push_back(ti);
+                    grp.coef.push_back(1.0f);
+                    placed = true;
+                    break;
+                }
+            if (!placed) {
+                FusedGroup grp;
+                grp.target = f;
+                grp.terms.push_back(ti);
+                grp.coef.push_back(1.0f);
+                grp.pres = pr;
+                gs.push_back(grp);
+            }
+            nprod++;
+        }
+        // fold groups whose lists are one monomial of the same shape: the
+        // coefficient moves into real space, the multiplier keeps c = 1
+        for (size_t a = 0; a < gs.size(); a++) {
+            if (!single_monomial(gs[a].pres)) continue;
+            for (size_t b = a + 1; b < gs.size();) {
+                if (single_monomial(gs[b].pres) && same_monomial(gs[a].pres[0], gs[b].pres[0])) {
+                    const float ca = gs[a].pres[0].preFactor, cb = gs[b].pres[0].preFactor;
+                    if (ca != 1.0f) {
+                        for (float &c : gs[a].coef) c *= ca;
+                        gs[a].pres[0].preFactor = 1.0f;
+                    }
+                    for (size_t k = 0; k < gs[b].terms.size(); k++) {
+                        gs[a].terms.push_back(gs[b].terms[k]);
+                        gs[a].coef.push_back(gs[b].coef[k] * cb);
+                    }
+                    gs.erase(gs.begin() + b);
+                } else {
+                    b++;
+                }
+            }
+        }
+        P.groups.insert(P.groups.end(), gs.begin(), gs.end());
+    }
+    if ((int)P.groups.size() > FUSED_MAX_GROUPS || nprod > FUSED_MAX_PRODS) return;
+    for (int i = 0; i < nf; i++)
         if (derived[i]) P.derived.push_back(i);
+    if ((int)P.derived.size() > FUSED_MAX_DERIVED) return;
+    // the prefactor pool must hold every multiplier of the instance
+    size_t pool = 0;
+    for (auto &grp : P.groups) pool += grp.pres.size();
+    for (int f : order) {
+        pool += s->f[f].imp.size();
+        for (int ti : s->f[f].terms)
+            if (s->t[ti].fac.size() < 2) pool += s->t[ti].pres.size();
+    }
+    for (int d : P.derived) pool += s->t[s->f[d].terms[0]].pres.size();
+    if (pool > (size_t)FUSED_MAX_POOL) return;
     P.phi = phi;
     P.ok = true;
 }
-
-
 
 extern "C" int cupss_system_time_advance(cupss_system s, int nsteps, float *ms)
 {
@@ -839,6 +926,7 @@ extern "C" int cupss_system_get_step(cupss_system s, uint64_t *step)
 extern "C" int cupss_system_set_step(cupss_system s, uint64_t step)
 {
     if (int r = check_sys(s)) return r;
+    s->cont_step = ~0ull;
     s->step = step;
     return CUPSS_OK;
 }

@@ -49,12 +49,23 @@ struct TermRec {
     float2 *xhat = nullptr;
 };
 
+// Product terms of one field transformed together: M * FFT(sum_t c_t prod_t)
+// (terms with equal prefactor lists, c_t = 1, or single-monomial prefactor
+// lists that differ only in the coefficient, folded into c_t).
+struct FusedGroup {
+    int target = -1;                  // field the terms belong to
+    std::vector<int> terms;
+    std::vector<float> coef;
+    std::vector<cupss_pres> pres;     // the group's multiplier
+};
+
 struct FusedPlan {
     bool ok = false;
     int phi = -1;
-    std::vector<int> real;                 // fields read in real space by the products
-    std::vector<std::vector<int>> groups;  // phi's product terms with equal prefactors
-    std::vector<int> derived;              // constraint fields psi = M_psi phi
+    std::vector<int> real;            // fields read in real space by the products
+    std::vector<FusedGroup> groups;   // product groups, phi's first, then by stored field
+    std::vector<int> derived;         // constraint fields psi = M_psi phi
+    std::vector<int> cons;            // stored constraint fields
 };
 
 }  // namespace cupss
@@ -73,9 +84,18 @@ struct cupss_system_t {
     int path_mode = 0;        // 0 = fused step when eligible, 1 = general step only
     int noise_mode = 0;       // 0 = real-space Philox + R2C, 1 = spectral-space Philox
     cupss::FusedPlan plan;
-    float2 *fz[cupss::FUSED_MAX_REAL] = {nullptr, nullptr, nullptr};
-    float2 *fx[cupss::FUSED_MAX_GROUPS] = {nullptr, nullptr, nullptr, nullptr};
+    float2 *fz[cupss::FUSED_MAX_REAL] = {};
+    float2 *fx[cupss::FUSED_MAX_GROUPS] = {};
     float2 *ft[2] = {nullptr, nullptr};  // phi's spectrum, column-major, during fused runs
+    float2 *fc[cupss::FUSED_MAX_CONS][2] = {};  // stored constraint fields, column-major
+    // After a fused run, fc[.][cont_buf] holds the stored fields' values of
+    // the next step's phase A/B; a run that starts at step cont_step with
+    // generation cont_gen continues from them.  Anything that changes the
+    // state in between (general steps, set_real, ...) resets cont_step.
+    uint64_t cont_step = ~0ull;
+    uint64_t cont_gen = 0;
+    int cont_buf = 0;
+    bool cont_now = false;                        // the current run continues
     long fused_steps = 0;     // steps taken by the fused path (diagnostics)
     uint64_t step = 0;
     int device = 0;
@@ -96,6 +116,8 @@ namespace cupss {
 int build_update(cupss_system s, int fi, UpdateArgs &a);
 int materialize(cupss_system s, int fi);
 int general_steps(cupss_system s, int nsteps);
+// phase A/B of the reference step (constraint fields) with the general program
+int general_constraint_phase(cupss_system s);
 // fused path (batch.hip)
 int fused_steps(cupss_system s, int nsteps);
 void fused_release(cupss_system s);

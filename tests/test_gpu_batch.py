@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import cupss_amd as C
-from tests.helpers import build_system, qgrid, spec_cahn_hilliard, spec_kpz
+from tests.helpers import build_system, qgrid, rel_l2, spec_cahn_hilliard, spec_kpz, spec_model_h
 
 pytestmark = pytest.mark.gpu
 
@@ -33,6 +33,9 @@ def mixed_specs():
         else:
             lam = 0.0 if i % 3 == 1 else float(rng.uniform(0, 2))
             specs.append(spec_kpz(n, nu=float(rng.uniform(0.5, 2)), lam=lam, D=0.01, dt=0.005, seed=100 + i))
+    # model H: stored constraint fields (sigma, v, w) beside derived ones
+    specs.append(spec_model_h(64, seed=41))
+    specs.append(spec_model_h(128, seed=42))
     return specs
 
 
@@ -125,3 +128,28 @@ def test_kpz_structure_factor_same_for_both_noise_draws():
     r = res[1][fast].sum() / res[0][fast].sum()
     print("  KPZ S(q) spectral / real-space draw: %.4f" % r)
     assert abs(r - 1.0) < 0.01
+
+
+def test_model_h_split_runs_match_one_run():
+    """Runs of 10 + 13 steps against one run of 23.  A run continues from
+    the stored fields' next phase that the previous fused run left: equal bit
+    for bit.  After a change of state in between (here set_real with the
+    same values) the next run rederives that phase with the general program:
+    equal to rounding."""
+    sp = spec_model_h(64, seed=43)
+    a, ids = make(sp)
+    b, _ = make(sp)
+    c, _ = make(sp)
+    assert a.fused_eligible
+    a.step(23)
+    b.step(10)
+    b.step(13)
+    c.step(10)
+    c.set_real(ids["phi"], c.get_real(ids["phi"]))
+    c.step(13)
+    for name, f in ids.items():
+        x, y, z = a.get_real(f), b.get_real(f), c.get_real(f)
+        err = rel_l2(z, x)
+        print("  %-7s split: max|diff| %.3e; rederived: rel L2 %.3e" % (name, float(np.max(np.abs(y - x))), err))
+        assert np.array_equal(x, y), name
+        assert err < 1e-5, name

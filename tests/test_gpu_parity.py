@@ -154,8 +154,11 @@ def test_reinitialise_keeps_stale_dealiased_copy(path):
     assert err < TOL
 
 
-def test_model_h():
-    compare(spec_model_h(64), 50, expect_fused=False)
+@pytest.mark.parametrize("path", [0, 1])
+def test_model_h(path):
+    """Stored constraint fields (sigma, v, w lag phi by one and two steps)
+    on the fused path, and the general program."""
+    compare(spec_model_h(64), 50, path=path, expect_fused=True if path == 0 else None)
 
 
 def test_constraint_division_noise_and_nyquist():
