@@ -97,6 +97,7 @@ WORKLOADS = {
                       cpu_steps=2000),
     "ensemble": dict(name="batched EW/KPZ ensemble, 512 instances of 64^2..512^2", cpu_steps=0),
     "serve": dict(name="served model-H stream, Poisson arrivals, sides 128..1024", cpu_steps=0),
+    "modelh": dict(build=None, dims=(1024, 1024, 1), name="2D model H 1024^2 (single instance)", cpu_steps=1),
 }
 
 
@@ -169,14 +170,22 @@ def bench_serve(args, C):
             srv.submit(lambda n=n, seed=seed: build(spec_model_h(n, seed=seed))[0], k, arrival=t,
                        bytes_estimate=est[n], done=sink)
 
-    # capacity: a saturated mix (two jobs of each size, all present at t = 0)
-    cal = [(n, 400, 7 + i) for i, n in enumerate(SERVE_SIZES * 2)]
-    srv = C.Server(budget, quantum=args.quantum, max_active=args.max_active)
-    submit(srv, cal, [0.0] * len(cal))
-    srv.run()
-    st = srv.stats()
-    srv.close()
-    capacity = st["point_steps"] / st["wall_s"]
+    # capacity: the batched step's throughput on a saturated mix (two jobs of
+    # each size advanced together, steps replayed from a hipGraph)
+    batch = C.Batch(graph_steps=args.quantum & ~1)
+    cal = []
+    for i, n in enumerate(SERVE_SIZES * 2):
+        s, _ = build(spec_model_h(n, seed=7 + i))
+        batch.add(s)
+        cal.append(s)
+    batch.step(2 * args.quantum)
+    batch.synchronize()
+    k = 4 * args.quantum
+    ms = batch.time_steps(k)
+    capacity = sum(n * n for n in SERVE_SIZES * 2) * k / (ms / 1e3)
+    batch.close()
+    for s in cal:
+        s.close()
     mean_points = sum(w * n * n for n, w in zip(SERVE_SIZES, SERVE_WEIGHTS))
     a, b = SERVE_STEPS
     mean_steps = (b - a) / np.log(b / a)
@@ -337,6 +346,12 @@ def main():
     dims = wl["dims"]
     n = dims[0] * dims[1] * dims[2]
     x0 = np.random.default_rng(1).uniform(-0.1, 0.1, n).astype(np.float32)
+    if args.workload == "modelh":
+        from cupss_amd.systems import build, spec_model_h
+        spec = spec_model_h(dims[0], seed=1)
+        spec["init"] = {"phi": x0}
+        wl["build"] = lambda S, P, dims, x0, **kw: build(dict(spec, **({"precision": kw["precision"]} if kw else {})),
+                                                        S, P)
     s, phi = wl["build"](C.System, C.pres, dims, x0)
     s.set_path(0 if args.path == "auto" else 1)
     path = "fused two-kernel step" if (args.path == "auto" and s.fused_eligible) else "general step program"
