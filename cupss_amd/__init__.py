@@ -81,6 +81,10 @@ _SIGS = {
     "cupss_batch_remove": (_I, [_VP, _VP]),
     "cupss_system_set_graph_steps": (_I, [_VP, _I]),
     "cupss_system_device_bytes": (_I, [_VP, ctypes.POINTER(ctypes.c_size_t)]),
+    "cupss_system_profile_kernel": (_I, [_VP, _I]),
+    "cupss_system_kernel_time": (_I, [_VP, _FP, ctypes.POINTER(ctypes.c_double), ctypes.c_char_p, _I]),
+    "cupss_batch_profile_kernel": (_I, [_VP, _I]),
+    "cupss_batch_kernel_time": (_I, [_VP, _FP, ctypes.POINTER(ctypes.c_double), ctypes.c_char_p, _I]),
     "cupss_server_create": (_I, [ctypes.c_size_t, _I, _I, ctypes.POINTER(_VP)]),
     "cupss_server_destroy": (_I, [_VP]),
     "cupss_server_submit": (_I, [_VP, ctypes.c_double, _I, ctypes.c_size_t, _VP, _VP, _VP, ctypes.POINTER(_I)]),
@@ -208,6 +212,15 @@ class System:
     def set_graph_steps(self, k):
         check(self.L.cupss_system_set_graph_steps(self.h, k))
 
+    def profile_kernel(self, on=True):
+        check(self.L.cupss_system_profile_kernel(self.h, int(on)))
+
+    def kernel_time(self):
+        """(average ms, real points per launch, name) of the timed column launches."""
+        ms, pts, name = ctypes.c_float(), ctypes.c_double(), ctypes.create_string_buffer(64)
+        check(self.L.cupss_system_kernel_time(self.h, ctypes.byref(ms), ctypes.byref(pts), name, 64))
+        return ms.value, pts.value, name.value.decode()
+
     def device_bytes(self):
         b = ctypes.c_size_t()
         check(self.L.cupss_system_device_bytes(self.h, ctypes.byref(b)))
@@ -307,6 +320,15 @@ class Batch:
             check(self.L.cupss_batch_set_graph_steps(self.h, graph_steps))
         if not autotune:
             check(self.L.cupss_batch_set_autotune(self.h, 0))
+
+    def profile_kernel(self, on=True):
+        check(self.L.cupss_batch_profile_kernel(self.h, int(on)))
+
+    def kernel_time(self):
+        """(average ms, real points per launch, name) of the timed column launches."""
+        ms, pts, name = ctypes.c_float(), ctypes.c_double(), ctypes.create_string_buffer(64)
+        check(self.L.cupss_batch_kernel_time(self.h, ctypes.byref(ms), ctypes.byref(pts), name, 64))
+        return ms.value, pts.value, name.value.decode()
 
     def schedule(self):
         """The schedule in use: size classes, tile shapes, chunks."""

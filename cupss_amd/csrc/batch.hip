@@ -89,6 +89,14 @@ struct FusedBatch {
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
 
+    // kernel timing: events around every column launch of the class with the
+    // most points (one chunk), recorded in the timed runs themselves (inside
+    // the graph when one is replayed); the last run's launches are read back
+    bool profile = false;
+    int pclass = -1;
+    std::vector<hipEvent_t> pev;
+    int pcount = 0;
+
     long steps_run = 0;
     std::vector<uint64_t> gens;   // system generations the tables were built for
 
@@ -132,6 +140,8 @@ struct FusedBatch {
     {
         if (stream) hipStreamSynchronize(stream);
         release_tables();
+        for (hipEvent_t e : pev) hipEventDestroy(e);
+        pev.clear();
         hipFree(d);
         d = nullptr;
         hipHostFree(pinned);
@@ -146,6 +156,8 @@ struct FusedBatch {
 
 // no transform needed: no products, no stored fields, no real-space noise,
 // and phi's other terms read phi itself or nothing
+static int retable(FusedBatch &B);
+
 static bool is_elementwise(const FusedInst &a)
 {
     if (a.ngroups != 0 || a.nreal != 0 || a.ncons != 0 || a.noise_mode == NOISE_REAL) return false;
@@ -453,6 +465,8 @@ static int build(FusedBatch &B)
     }
     if (int r = upload_blocks(er, &B.eb, &B.neb)) return r;
     if (int r = upload_blocks(tiles, &B.tb, &B.ntb)) return r;
+    if (B.profile)
+        if (int r = retable(B)) return r;
     B.gens.clear();
     for (cupss_system s : B.sys) B.gens.push_back(s->gen);
     B.built = true;
@@ -492,13 +506,35 @@ static int launch_prologue(FusedBatch &B)
 // reads spectrum buffer n&1 and uses counter offset n - rel0
 static int launch_steps(FusedBatch &B, int n0, int k, int rel0)
 {
-    for (auto &c : B.classes)
+    if (B.profile) B.pcount = 0;
+    for (size_t ci = 0; ci < B.classes.size(); ci++) {
+        auto &c = B.classes[ci];
+        const bool timed = B.profile && (int)ci == B.pclass && c.chunks.size() == 1;
         for (auto &t : c.chunks)
             for (int n = n0; n < n0 + k; n++) {
                 HIP_TRY(fused_rows_launch(c.nc, c.lpb, c.wide, B.d, t.rb, t.nrb, (uint32_t)(n - rel0), B.stream));
+                const bool ev = timed && 2 * B.pcount + 1 < (int)B.pev.size();
+                if (ev) HIP_TRY(hipEventRecord(B.pev[2 * B.pcount], B.stream));
                 HIP_TRY(fused_cols_launch(c.ny, c.twd, B.d, t.cb, t.ncb, n & 1, (uint32_t)(n - rel0), 1, B.stream));
+                if (ev) HIP_TRY(hipEventRecord(B.pev[2 * B.pcount++ + 1], B.stream));
             }
+    }
     return CUPSS_OK;
+}
+
+// the class the timing events follow: the most real points
+static void pick_profile_class(FusedBatch &B)
+{
+    B.pclass = -1;
+    double best = -1;
+    for (size_t ci = 0; ci < B.classes.size(); ci++) {
+        double pts = 0;
+        for (int i : B.classes[ci].insts) pts += (double)B.h[i].g.n;
+        if (pts > best) {
+            best = pts;
+            B.pclass = (int)ci;
+        }
+    }
 }
 
 // ---- autotune
@@ -609,11 +645,22 @@ static int tune(FusedBatch &B)
     }
     HIP_TRY(hipStreamSynchronize(B.stream));
     B.chunk_bytes = pick;
-    for (auto &c : B.classes) {
-        FusedBatch::free_tables(c.chunks);
-        if (int r = class_tables(B, c, c.lpb, c.twd, B.chunk_bytes, c.chunks)) return r;
-    }
     B.tuned = true;
+    return retable(B);
+}
+
+// block tables of every class for its shapes and the chunk target (the
+// class being timed stays one chunk)
+static int retable(FusedBatch &B)
+{
+    HIP_TRY(hipStreamSynchronize(B.stream));
+    if (B.profile) pick_profile_class(B);
+    for (size_t ci = 0; ci < B.classes.size(); ci++) {
+        auto &c = B.classes[ci];
+        FusedBatch::free_tables(c.chunks);
+        const double cb = B.profile && (int)ci == B.pclass ? 0.0 : B.chunk_bytes;
+        if (int r = class_tables(B, c, c.lpb, c.twd, cb, c.chunks)) return r;
+    }
     return CUPSS_OK;
 }
 
@@ -847,6 +894,67 @@ extern "C" int cupss_batch_schedule(cupss_batch b, char *buf, int len)
     const std::string t = schedule_text(b->b);
     std::snprintf(buf, (size_t)len, "%s", t.c_str());
     return CUPSS_OK;
+}
+
+static int set_profile(FusedBatch &B, int on)
+{
+    B.profile = on != 0;
+    B.drop_graph();
+    if (B.profile && B.pev.empty()) {
+        B.pev.resize(2 * 64);
+        for (auto &e : B.pev) HIP_TRY(hipEventCreate(&e));
+    }
+    if (B.built)
+        if (int r = retable(B)) return r;
+    return CUPSS_OK;
+}
+
+static int kernel_time(FusedBatch &B, float *avg_ms, double *points, char *name, int len)
+{
+    if (!B.profile || B.pclass < 0 || B.pcount == 0) return fail(CUPSS_ERR_STATE, "no timed launches yet");
+    HIP_TRY(hipStreamSynchronize(B.stream));
+    double sum = 0;
+    for (int j = 0; j < B.pcount; j++) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, B.pev[2 * j], B.pev[2 * j + 1]));
+        sum += ms;
+    }
+    const auto &c = B.classes[B.pclass];
+    double pts = 0;
+    for (int i : c.insts) pts += (double)B.h[i].g.n;
+    if (avg_ms) *avg_ms = (float)(sum / B.pcount);
+    if (points) *points = pts;
+    if (name && len > 0) std::snprintf(name, (size_t)len, "k_fcols<%d, %d>", c.ny, c.twd);
+    return CUPSS_OK;
+}
+
+int cupss::fused_profile(cupss_system s, int on)
+{
+    if (!s->batch) {
+        FusedBatch *B = new FusedBatch();
+        B->sys.push_back(s);
+        B->stream = s->stream;
+        s->batch = B;
+    }
+    return set_profile(*static_cast<FusedBatch *>(s->batch), on);
+}
+
+int cupss::fused_kernel_time(cupss_system s, float *avg_ms, double *points, char *name, int len)
+{
+    if (!s->batch) return fail(CUPSS_ERR_STATE, "the system has not run the fused path");
+    return kernel_time(*static_cast<FusedBatch *>(s->batch), avg_ms, points, name, len);
+}
+
+extern "C" int cupss_batch_profile_kernel(cupss_batch b, int on)
+{
+    if (!b) return fail(CUPSS_ERR_INVALID, "null batch");
+    return set_profile(b->b, on);
+}
+
+extern "C" int cupss_batch_kernel_time(cupss_batch b, float *avg_ms, double *points, char *name, int len)
+{
+    if (!b) return fail(CUPSS_ERR_INVALID, "null batch");
+    return kernel_time(b->b, avg_ms, points, name, len);
 }
 
 extern "C" int cupss_batch_advance(cupss_batch b, int nsteps)

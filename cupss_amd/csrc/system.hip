@@ -702,6 +702,18 @@ extern "C" int cupss_system_set_graph_steps(cupss_system s, int k)
     return CUPSS_OK;
 }
 
+extern "C" int cupss_system_profile_kernel(cupss_system s, int on)
+{
+    if (int r = check_sys(s)) return r;
+    return fused_profile(s, on);
+}
+
+extern "C" int cupss_system_kernel_time(cupss_system s, float *avg_ms, double *points, char *name, int len)
+{
+    if (int r = check_sys(s)) return r;
+    return fused_kernel_time(s, avg_ms, points, name, len);
+}
+
 extern "C" int cupss_system_device_bytes(cupss_system s, size_t *bytes)
 {
     if (int r = check_sys(s)) return r;

@@ -121,4 +121,6 @@ int general_constraint_phase(cupss_system s);
 // fused path (batch.hip)
 int fused_steps(cupss_system s, int nsteps);
 void fused_release(cupss_system s);
+int fused_profile(cupss_system s, int on);
+int fused_kernel_time(cupss_system s, float *avg_ms, double *points, char *name, int len);
 }  // namespace cupss

@@ -152,6 +152,9 @@ int cupss_system_set_graph_steps(cupss_system s, int k);
 /* device bytes the system holds, including the fused path's buffers it
  * allocates at its first fused run (for admission by memory budget) */
 int cupss_system_device_bytes(cupss_system s, size_t *bytes);
+/* the same kernel timing for a system's own fused runs */
+int cupss_system_profile_kernel(cupss_system s, int on);
+int cupss_system_kernel_time(cupss_system s, float *avg_ms, double *points, char *name, int len);
 /* evolver::dt change is not part of the reference API; exposed for sweeps */
 int cupss_system_set_dt(cupss_system s, float dt);
 
@@ -183,6 +186,13 @@ int cupss_batch_set_autotune(cupss_batch b, int on);
  * chunks) into buf (len bytes, NUL-terminated) */
 int cupss_batch_schedule(cupss_batch b, char *buf, int len);
 int cupss_batch_advance(cupss_batch b, int nsteps);
+/* Kernel timing for roofline figures: with `on`, every column launch of the
+ * size class with the most points is bracketed by HIP events on the batch
+ * stream (inside the replayed graph too); cupss_batch_kernel_time returns
+ * the average duration of those launches in the last run, the real points
+ * one launch covers and the kernel's name. */
+int cupss_batch_profile_kernel(cupss_batch b, int on);
+int cupss_batch_kernel_time(cupss_batch b, float *avg_ms, double *points, char *name, int len);
 int cupss_batch_time_advance(cupss_batch b, int nsteps, float *ms);
 int cupss_batch_synchronize(cupss_batch b);
 
