@@ -89,13 +89,15 @@ struct FusedBatch {
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
 
-    // kernel timing: events around every column launch of the class with the
-    // most points (one chunk), recorded in the timed runs themselves (inside
-    // the graph when one is replayed); the last run's launches are read back
+    // kernel timing: every column launch of the class with the most points
+    // (one chunk) records its earliest workgroup start and latest end on the
+    // device wall clock, inside the timed runs themselves (and the replayed
+    // graphs); the last run's launches are read back
     bool profile = false;
     int pclass = -1;
-    std::vector<hipEvent_t> pev;
+    unsigned long long *pts = nullptr;  // timestamp slots of up to PROF_SLOTS timed launches
     int pcount = 0;
+    static constexpr int PROF_SLOTS = 64;
 
     long steps_run = 0;
     std::vector<uint64_t> gens;   // system generations the tables were built for
@@ -140,8 +142,8 @@ struct FusedBatch {
     {
         if (stream) hipStreamSynchronize(stream);
         release_tables();
-        for (hipEvent_t e : pev) hipEventDestroy(e);
-        pev.clear();
+        hipFree(pts);
+        pts = nullptr;
         hipFree(d);
         d = nullptr;
         hipHostFree(pinned);
@@ -506,17 +508,21 @@ static int launch_prologue(FusedBatch &B)
 // reads spectrum buffer n&1 and uses counter offset n - rel0
 static int launch_steps(FusedBatch &B, int n0, int k, int rel0)
 {
-    if (B.profile) B.pcount = 0;
+    if (B.profile) {
+        B.pcount = 0;
+        HIP_TRY(fused_ts_reset_launch(B.pts, FusedBatch::PROF_SLOTS, B.stream));
+    }
     for (size_t ci = 0; ci < B.classes.size(); ci++) {
         auto &c = B.classes[ci];
         const bool timed = B.profile && (int)ci == B.pclass && c.chunks.size() == 1;
         for (auto &t : c.chunks)
             for (int n = n0; n < n0 + k; n++) {
                 HIP_TRY(fused_rows_launch(c.nc, c.lpb, c.wide, B.d, t.rb, t.nrb, (uint32_t)(n - rel0), B.stream));
-                const bool ev = timed && 2 * B.pcount + 1 < (int)B.pev.size();
-                if (ev) HIP_TRY(hipEventRecord(B.pev[2 * B.pcount], B.stream));
-                HIP_TRY(fused_cols_launch(c.ny, c.twd, B.d, t.cb, t.ncb, n & 1, (uint32_t)(n - rel0), 1, B.stream));
-                if (ev) HIP_TRY(hipEventRecord(B.pev[2 * B.pcount++ + 1], B.stream));
+                unsigned long long *ts = nullptr;
+                if (timed && B.pcount < FusedBatch::PROF_SLOTS)
+                    ts = B.pts + (size_t)(B.pcount++) * FUSED_TS_SLOTS * FUSED_TS_STRIDE;
+                HIP_TRY(fused_cols_launch(c.ny, c.twd, B.d, t.cb, t.ncb, n & 1, (uint32_t)(n - rel0), 1, B.stream,
+                                          ts));
             }
     }
     return CUPSS_OK;
@@ -900,10 +906,10 @@ static int set_profile(FusedBatch &B, int on)
 {
     B.profile = on != 0;
     B.drop_graph();
-    if (B.profile && B.pev.empty()) {
-        B.pev.resize(2 * 64);
-        for (auto &e : B.pev) HIP_TRY(hipEventCreate(&e));
-    }
+    if (B.profile && !B.pts &&
+        hipMalloc(&B.pts, sizeof(unsigned long long) * FusedBatch::PROF_SLOTS * FUSED_TS_SLOTS * FUSED_TS_STRIDE) !=
+            hipSuccess)
+        return fail(CUPSS_ERR_NOMEM, "device allocation failed");
     if (B.built)
         if (int r = retable(B)) return r;
     return CUPSS_OK;
@@ -913,11 +919,28 @@ static int kernel_time(FusedBatch &B, float *avg_ms, double *points, char *name,
 {
     if (!B.profile || B.pclass < 0 || B.pcount == 0) return fail(CUPSS_ERR_STATE, "no timed launches yet");
     HIP_TRY(hipStreamSynchronize(B.stream));
+    const size_t per = (size_t)FUSED_TS_SLOTS * FUSED_TS_STRIDE;
+    std::vector<unsigned long long> raw(per * B.pcount), h(2 * B.pcount);
+    HIP_TRY(hipMemcpy(raw.data(), B.pts, sizeof(unsigned long long) * raw.size(), hipMemcpyDeviceToHost));
+    for (int j = 0; j < B.pcount; j++) {
+        unsigned long long lo = ~0ull, hi = 0;
+        for (int q = 0; q < FUSED_TS_SLOTS; q++) {
+            const unsigned long long *sl = raw.data() + j * per + (size_t)q * FUSED_TS_STRIDE;
+            if (sl[1] == 0) continue;  // no workgroup of this launch used the slot
+            lo = std::min(lo, sl[0]);
+            hi = std::max(hi, sl[1]);
+        }
+        h[2 * j] = lo;
+        h[2 * j + 1] = hi;
+    }
+    int dev = 0, khz = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    if (khz <= 0) return fail(CUPSS_ERR_HIP, "no wall-clock rate");
     double sum = 0;
     for (int j = 0; j < B.pcount; j++) {
-        float ms = 0;
-        HIP_TRY(hipEventElapsedTime(&ms, B.pev[2 * j], B.pev[2 * j + 1]));
-        sum += ms;
+        if (h[2 * j + 1] < h[2 * j]) return fail(CUPSS_ERR_STATE, "timed launch did not record");
+        sum += (double)(h[2 * j + 1] - h[2 * j]) / (double)khz;
     }
     const auto &c = B.classes[B.pclass];
     double pts = 0;

@@ -316,7 +316,8 @@ template <int N, int TWD>
 __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const FusedInst *__restrict__ insts,
                                                                    const FusedBlock *__restrict__ blocks,
                                                                    int parity, uint32_t step_rel, int mode,
-                                                                   const float2 *__restrict__ sets)
+                                                                   const float2 *__restrict__ sets,
+                                                                   unsigned long long *__restrict__ ts)
 {
     using C = FColCfg<N, TWD>;
     constexpr int E = C::E, T = C::T;
@@ -324,6 +325,11 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
     __shared__ float2 lds[TWD * C::LL];
     const FusedBlock blk = blocks[blockIdx.x];
     if (blk.inst < 0) return;
+    // launch timing (ts != null): earliest workgroup start and latest end in
+    // ticks of the constant-rate wall clock, spread over FUSED_TS_SLOTS
+    // cache lines so that the workgroups' atomics do not contend
+    unsigned long long *slot = ts ? ts + (size_t)(blockIdx.x % FUSED_TS_SLOTS) * FUSED_TS_STRIDE : nullptr;
+    if (slot && threadIdx.x == 0) atomicMin(&slot[0], (unsigned long long)wall_clock64());
     const FusedInst &a = insts[blk.inst];
     const Geom &g = a.g;
     const int c = threadIdx.x / T, t = threadIdx.x % T;
@@ -458,6 +464,10 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
             for (int e = 0; e < E; e++) v[e] = nw[e];
         }
         z_out<N, E, T>(a, f, g, cb, v, !upd && a.zero[f] != 0, blk.line, active, col, L, t, tws);
+    }
+    if (slot) {
+        __syncthreads();
+        if (threadIdx.x == 0) atomicMax(&slot[1], (unsigned long long)wall_clock64());
     }
 }
 
@@ -745,13 +755,13 @@ hipError_t fused_rows_launch(int nc, int lpb, bool wide, const FusedInst *inst, 
 
 template <int N>
 static hipError_t fcols_n(int twd, const FusedInst *inst, const FusedBlock *blocks, int nblocks, int parity,
-                          uint32_t step_rel, int mode, hipStream_t s, const float2 *sets)
+                          uint32_t step_rel, int mode, hipStream_t s, const float2 *sets, unsigned long long *ts)
 {
 #define X(TWD)                                                                                                \
     if constexpr (FColCfg<N, TWD>::valid()) {                                                                \
         if (twd == TWD) {                                                                                     \
             hipLaunchKernelGGL((k_fcols<N, TWD>), dim3(nblocks), dim3(FColCfg<N, TWD>::THREADS), 0, s, inst, \
-                               blocks, parity, step_rel, mode, sets);                                         \
+                               blocks, parity, step_rel, mode, sets, ts);                                     \
             return hipGetLastError();                                                                         \
         }                                                                                                     \
     }
@@ -761,14 +771,14 @@ static hipError_t fcols_n(int twd, const FusedInst *inst, const FusedBlock *bloc
 }
 
 hipError_t fused_cols_launch(int ny, int twd, const FusedInst *inst, const FusedBlock *blocks, int nblocks,
-                             int parity, uint32_t step_rel, int mode, hipStream_t s)
+                             int parity, uint32_t step_rel, int mode, hipStream_t s, unsigned long long *ts)
 {
     const float2 *sets = twiddle_sets(s);
     if (!sets) return hipErrorOutOfMemory;
     switch (ny) {
 #define X(N) \
     case N:  \
-        return fcols_n<N>(twd, inst, blocks, nblocks, parity, step_rel, mode, s, sets);
+        return fcols_n<N>(twd, inst, blocks, nblocks, parity, step_rel, mode, s, sets, ts);
         FUSED_COL_CASES(X)
 #undef X
     default:
@@ -793,6 +803,26 @@ hipError_t fused_elem_launch(const FusedInst *inst, const FusedBlock *blocks, in
 {
     hipLaunchKernelGGL(k_felem, dim3(nblocks), dim3(FUSED_ELEM_BLOCK), 0, s, inst, blocks, parity, step_rel,
                        nsteps);
+    return hipGetLastError();
+}
+
+}  // namespace cupss
+
+namespace cupss {
+
+__global__ void k_ts_reset(unsigned long long *ts, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n * FUSED_TS_SLOTS) {
+        ts[(size_t)i * FUSED_TS_STRIDE] = ~0ull;
+        ts[(size_t)i * FUSED_TS_STRIDE + 1] = 0ull;
+    }
+}
+
+hipError_t fused_ts_reset_launch(unsigned long long *ts, int n, hipStream_t s)
+{
+    const int m = n * FUSED_TS_SLOTS;
+    hipLaunchKernelGGL(k_ts_reset, dim3((m + 255) / 256), dim3(256), 0, s, ts, n);
     return hipGetLastError();
 }
 

@@ -29,6 +29,8 @@ constexpr int FUSED_MAX_POOL = 96;    // prefactors of all multipliers of an ins
 constexpr int FUSED_MAX_DERIVED = 4;  // derived fields written back after a run
 constexpr int FUSED_TILE = 32;        // side of the transpose tiles
 constexpr int FUSED_ELEM_BLOCK = 256; // bin pairs per workgroup of the elementwise kernel
+constexpr int FUSED_TS_SLOTS = 64;    // timestamp slots per timed launch
+constexpr int FUSED_TS_STRIDE = 16;   // u64 between slots (one 128-byte line each)
 
 enum NoiseMode : int {
     NOISE_NONE = 0,
@@ -123,8 +125,13 @@ std::vector<int> fused_col_shapes(int ny);
 hipError_t fused_rows_launch(int nc, int lpb, bool wide, const FusedInst *inst, const FusedBlock *blocks,
                              int nblocks, uint32_t step_rel, hipStream_t s);
 // mode 0: prologue (real fields of the current state), 1: step
+// (ts: null, or FUSED_TS_SLOTS pairs {earliest start, latest end} of
+// wall-clock ticks, FUSED_TS_STRIDE apart, that the launch's workgroups
+// min/max-update; the launch spans min(starts) .. max(ends))
 hipError_t fused_cols_launch(int ny, int twd, const FusedInst *inst, const FusedBlock *blocks, int nblocks,
-                             int parity, uint32_t step_rel, int mode, hipStream_t s);
+                             int parity, uint32_t step_rel, int mode, hipStream_t s, unsigned long long *ts = nullptr);
+// the slots of n timed launches to {~0, 0} (a kernel, so that graphs can hold it)
+hipError_t fused_ts_reset_launch(unsigned long long *ts, int n, hipStream_t s);
 // natural spectra -> column-major buffer 0 (phi and the stored constraints;
 // blocks: one tile each)
 hipError_t fused_transpose_in_launch(const FusedInst *inst, const FusedBlock *blocks, int nblocks, hipStream_t s);

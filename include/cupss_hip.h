@@ -187,10 +187,11 @@ int cupss_batch_set_autotune(cupss_batch b, int on);
 int cupss_batch_schedule(cupss_batch b, char *buf, int len);
 int cupss_batch_advance(cupss_batch b, int nsteps);
 /* Kernel timing for roofline figures: with `on`, every column launch of the
- * size class with the most points is bracketed by HIP events on the batch
- * stream (inside the replayed graph too); cupss_batch_kernel_time returns
- * the average duration of those launches in the last run, the real points
- * one launch covers and the kernel's name. */
+ * size class with the most points records its earliest workgroup start and
+ * latest workgroup end on the device's constant-rate wall clock (inside the
+ * replayed graph too); cupss_batch_kernel_time returns the average duration
+ * of those launches in the last run, the real points one launch covers and
+ * the kernel's name. */
 int cupss_batch_profile_kernel(cupss_batch b, int on);
 int cupss_batch_kernel_time(cupss_batch b, float *avg_ms, double *points, char *name, int len);
 int cupss_batch_time_advance(cupss_batch b, int nsteps, float *ms);
