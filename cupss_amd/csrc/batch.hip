@@ -99,6 +99,14 @@ struct FusedBatch {
     int pcount = 0;
     static constexpr int PROF_SLOTS = 64;
 
+    // independent work (size classes, the elementwise set) runs on up to
+    // MAX_LANES streams: the batch stream and side streams forked from it
+    // and joined back with events (inside captured graphs as well)
+    static constexpr int MAX_LANES = 4;
+    std::vector<hipStream_t> lanes;
+    std::vector<hipEvent_t> ljoin;
+    hipEvent_t lfork = nullptr;
+
     long steps_run = 0;
     std::vector<uint64_t> gens;   // system generations the tables were built for
 
@@ -144,6 +152,12 @@ struct FusedBatch {
         release_tables();
         hipFree(pts);
         pts = nullptr;
+        for (hipStream_t l : lanes) hipStreamDestroy(l);
+        for (hipEvent_t e : ljoin) hipEventDestroy(e);
+        lanes.clear();
+        ljoin.clear();
+        if (lfork) hipEventDestroy(lfork);
+        lfork = nullptr;
         hipFree(d);
         d = nullptr;
         hipHostFree(pinned);
@@ -506,24 +520,89 @@ static int launch_prologue(FusedBatch &B)
 
 // steps [n0, n0+k) of every transform instance, chunk by chunk; step n
 // reads spectrum buffer n&1 and uses counter offset n - rel0
+// steps [n0, n0+k) of the instances of class ci, chunk by chunk, on stream st
+static int launch_class(FusedBatch &B, size_t ci, int n0, int k, int rel0, hipStream_t st)
+{
+    auto &c = B.classes[ci];
+    const bool timed = B.profile && (int)ci == B.pclass && c.chunks.size() == 1;
+    for (auto &t : c.chunks)
+        for (int n = n0; n < n0 + k; n++) {
+            HIP_TRY(fused_rows_launch(c.nc, c.lpb, c.wide, B.d, t.rb, t.nrb, (uint32_t)(n - rel0), st));
+            unsigned long long *ts = nullptr;
+            if (timed && B.pcount < FusedBatch::PROF_SLOTS)
+                ts = B.pts + (size_t)(B.pcount++) * FUSED_TS_SLOTS * FUSED_TS_STRIDE;
+            HIP_TRY(fused_cols_launch(c.ny, c.twd, B.d, t.cb, t.ncb, n & 1, (uint32_t)(n - rel0), 1, st, ts));
+        }
+    return CUPSS_OK;
+}
+
+// steps [n0, n0+k) of every transform instance on the batch stream; step n
+// reads spectrum buffer n&1 and uses counter offset n - rel0
 static int launch_steps(FusedBatch &B, int n0, int k, int rel0)
 {
     if (B.profile) {
         B.pcount = 0;
         HIP_TRY(fused_ts_reset_launch(B.pts, FusedBatch::PROF_SLOTS, B.stream));
     }
+    for (size_t ci = 0; ci < B.classes.size(); ci++)
+        if (int r = launch_class(B, ci, n0, k, rel0, B.stream)) return r;
+    return CUPSS_OK;
+}
+
+// steps [n0, n0+k) of every instance, the size classes and the elementwise
+// set spread over lanes (longest work first onto the least loaded lane)
+static int launch_all(FusedBatch &B, int n0, int k, int rel0)
+{
+    if (B.profile) {
+        B.pcount = 0;
+        HIP_TRY(fused_ts_reset_launch(B.pts, FusedBatch::PROF_SLOTS, B.stream));
+    }
+    std::vector<std::pair<double, int>> items;  // (cost, class index; -1 = elementwise)
     for (size_t ci = 0; ci < B.classes.size(); ci++) {
-        auto &c = B.classes[ci];
-        const bool timed = B.profile && (int)ci == B.pclass && c.chunks.size() == 1;
-        for (auto &t : c.chunks)
-            for (int n = n0; n < n0 + k; n++) {
-                HIP_TRY(fused_rows_launch(c.nc, c.lpb, c.wide, B.d, t.rb, t.nrb, (uint32_t)(n - rel0), B.stream));
-                unsigned long long *ts = nullptr;
-                if (timed && B.pcount < FusedBatch::PROF_SLOTS)
-                    ts = B.pts + (size_t)(B.pcount++) * FUSED_TS_SLOTS * FUSED_TS_STRIDE;
-                HIP_TRY(fused_cols_launch(c.ny, c.twd, B.d, t.cb, t.ncb, n & 1, (uint32_t)(n - rel0), 1, B.stream,
-                                          ts));
-            }
+        double cost = 0;
+        for (int i : B.classes[ci].insts) cost += (double)B.h[i].g.n * (4 + B.h[i].nreal + B.h[i].ngroups);
+        items.push_back({cost, (int)ci});
+    }
+    if (B.neb > 0) {
+        double cost = 0;
+        for (int i : B.elem) cost += (double)B.h[i].g.n;
+        items.push_back({cost, -1});
+    }
+    const int L = std::min<int>(FusedBatch::MAX_LANES, (int)items.size());
+    std::sort(items.begin(), items.end(), [](const std::pair<double, int> &a, const std::pair<double, int> &b) {
+        return a.first > b.first;
+    });
+    std::vector<double> load(std::max(L, 1), 0.0);
+    std::vector<int> lane(items.size(), 0);
+    for (size_t it = 0; it < items.size(); it++) {
+        const int j = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        lane[it] = j;
+        load[j] += items[it].first;
+    }
+    while ((int)B.lanes.size() < L - 1) {
+        hipStream_t st;
+        hipEvent_t ev;
+        HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        B.lanes.push_back(st);
+        B.ljoin.push_back(ev);
+    }
+    if (L > 1) {
+        if (!B.lfork) HIP_TRY(hipEventCreateWithFlags(&B.lfork, hipEventDisableTiming));
+        HIP_TRY(hipEventRecord(B.lfork, B.stream));
+        for (int j = 1; j < L; j++) HIP_TRY(hipStreamWaitEvent(B.lanes[j - 1], B.lfork, 0));
+    }
+    for (size_t it = 0; it < items.size(); it++) {
+        hipStream_t st = lane[it] == 0 ? B.stream : B.lanes[lane[it] - 1];
+        if (items[it].second >= 0) {
+            if (int r = launch_class(B, (size_t)items[it].second, n0, k, rel0, st)) return r;
+        } else {
+            HIP_TRY(fused_elem_launch(B.d, B.eb, B.neb, n0 & 1, (uint32_t)(n0 - rel0), k, st));
+        }
+    }
+    for (int j = 1; j < L; j++) {
+        HIP_TRY(hipEventRecord(B.ljoin[j - 1], B.lanes[j - 1]));
+        HIP_TRY(hipStreamWaitEvent(B.stream, B.ljoin[j - 1], 0));
     }
     return CUPSS_OK;
 }
@@ -740,11 +819,7 @@ static int run(FusedBatch &B, int nsteps)
     if (K >= 2 && nsteps >= K) {
         if (!B.gexec) {
             HIP_TRY(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
-            int rc = launch_steps(B, 0, K, 0);
-            if (rc == CUPSS_OK && B.neb > 0) {
-                const hipError_t e = fused_elem_launch(B.d, B.eb, B.neb, 0, 0, K, B.stream);
-                if (e != hipSuccess) rc = fail(CUPSS_ERR_HIP, hipGetErrorString(e));
-            }
+            int rc = launch_all(B, 0, K, 0);
             if (rc == CUPSS_OK) {
                 const int ni = (int)B.h.size();
                 hipLaunchKernelGGL(k_bump_steps, dim3((ni + 255) / 256), dim3(256), 0, B.stream, B.d, ni,
@@ -767,8 +842,7 @@ static int run(FusedBatch &B, int nsteps)
         for (size_t i = 0; i < B.h.size(); i++) B.h[i].step_base += (uint64_t)done;
     }
     if (nsteps > done) {
-        if (int r = launch_steps(B, done, nsteps - done, done)) return r;
-        if (B.neb > 0) HIP_TRY(fused_elem_launch(B.d, B.eb, B.neb, done & 1, 0, nsteps - done, B.stream));
+        if (int r = launch_all(B, done, nsteps - done, done)) return r;
     }
     B.steps_run += nsteps;
     return settle(B, nsteps);
