@@ -326,6 +326,8 @@ def main():
     ap.add_argument("--budget-gib", type=float, default=4.0, help="serve: device-memory budget")
     ap.add_argument("--quantum", type=int, default=100, help="serve: steps per scheduling round")
     ap.add_argument("--max-active", type=int, default=64, help="serve: resident jobs at most")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device (the known-size kernel that HBM counters are scaled by)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build runs on a single GPU (--gpus 1); "
@@ -334,6 +336,12 @@ def main():
 
     import cupss_amd as C
     wl = WORKLOADS[args.workload]
+    if args.copy_calibration > 0:
+        nb = args.copy_calibration << 20
+        src, dst = C.DeviceBuffer(nb), C.DeviceBuffer(nb)
+        C.check(C.lib().cupss_memcpy_d2d(dst.ptr, src.ptr, nb))
+        C.check(C.lib().cupss_device_synchronize())
+        del src, dst
     if args.workload == "serve":
         value, ms, extra, jobs = bench_serve(args, C)
         achieved = BYTES_PER_POINT_STEP * value / 1e9

@@ -101,6 +101,7 @@ _SIGS = {
     "cupss_device_free": (_I, [_VP]),
     "cupss_memcpy_h2d": (_I, [_VP, _VP, ctypes.c_size_t]),
     "cupss_memcpy_d2h": (_I, [_VP, _VP, ctypes.c_size_t]),
+    "cupss_memcpy_d2d": (_I, [_VP, _VP, ctypes.c_size_t]),
     "cupss_device_synchronize": (_I, []),
     "cupss_device_count": (_I, [ctypes.POINTER(_I)]),
 }

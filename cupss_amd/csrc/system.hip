@@ -1043,6 +1043,12 @@ extern "C" int cupss_memcpy_d2h(void *h, const void *d, size_t bytes)
     HIP_TRY(hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost));
     return CUPSS_OK;
 }
+extern "C" int cupss_memcpy_d2d(void *d, const void *s, size_t bytes)
+{
+    HIP_TRY(hipMemcpy(d, s, bytes, hipMemcpyDeviceToDevice));
+    return CUPSS_OK;
+}
+
 extern "C" int cupss_device_synchronize(void)
 {
     HIP_TRY(hipDeviceSynchronize());

@@ -247,6 +247,9 @@ int cupss_device_alloc(size_t bytes, void **dev_ptr);
 int cupss_device_free(void *dev_ptr);
 int cupss_memcpy_h2d(void *dev_dst, const void *host_src, size_t bytes);
 int cupss_memcpy_d2h(void *host_dst, const void *dev_src, size_t bytes);
+/* device-to-device copy (also the known-size reference kernel that HBM
+ * counter readings are scaled by) */
+int cupss_memcpy_d2d(void *dev_dst, const void *dev_src, size_t bytes);
 int cupss_device_synchronize(void);
 int cupss_device_count(int *n);
 
