@@ -13,6 +13,7 @@
 
 #include "../../../include/cupss.h"
 #include "../../../include/cupss_hip.h"
+#include "output.hpp"
 
 namespace {
 
@@ -106,6 +107,7 @@ void evolver::common_constructor()
 
 evolver::~evolver()
 {
+    cupss_host::output_flush();  // files of this system are complete when it is gone
     for (field *f : fields) delete f;
     delete _parser;
     cupss_system_destroy(device_system);

@@ -10,6 +10,7 @@
 
 #include "../../../include/cupss.h"
 #include "../../../include/cupss_hip.h"
+#include "output.hpp"
 
 namespace {
 
@@ -200,48 +201,33 @@ void field::stepRK2(float) { std::cout << "RK2 not implemented" << std::endl; }
 void field::stepRK4(float) { std::cout << "RK4 not implemented" << std::endl; }
 
 // field::writeToFile (src/field.cpp:350-402): data/<name>.csv.<step>,
-// header "x, [y, [z, ]]name", rows "%i, ... %.<precision>f"; NaN aborts.
+// header "x, [y, [z, ]]name", rows "%i, ... %.<precision>f"; NaN aborts
+// (checked here); the file itself is written by the background writer
+// (output.hpp).
 void field::writeToFile(int currentTimeStep, int dim, int writePrecision)
 {
     if (!outputToFile) return;
     if (isCUDA) copyRealDeviceToHost();
-    const std::string fileName = "data/" + name + ".csv." + std::to_string(currentTimeStep);
-    FILE *fp = fopen(fileName.c_str(), "w+");
-    if (fp == NULL) {
-        std::cout << "Error creating output file at timestep" << currentTimeStep << std::endl;
-        std::exit(1);
+    const size_t n = (size_t)sx * sy * sz;
+    cupss_host::OutputJob job;
+    job.file = "data/" + name + ".csv." + std::to_string(currentTimeStep);
+    job.name = name;
+    job.dim = dim;
+    job.sx = sx;
+    job.sy = sy;
+    job.sz = sz;
+    job.precision = writePrecision;
+    job.values.resize(n);
+    for (size_t i = 0; i < n; i++) {
+        const float v = real_array[i].x;
+        if (std::isnan(v)) {
+            std::cout << "NaN found in field " << name << std::endl;
+            std::exit(1);
+        }
+        job.values[i] = v;
     }
-    std::string fmt = "%i, ";
-    fprintf(fp, "x, ");
-    if (dim == 2) {
-        fprintf(fp, "y, ");
-        fmt += "%i, ";
-    }
-    if (dim == 3) {
-        fprintf(fp, "y, z, ");
-        fmt += "%i, %i, ";
-    }
-    fprintf(fp, "%s\n", name.c_str());
-    fmt += "%." + std::to_string(writePrecision) + "f\n";
-    for (int k = 0; k < sz; k++)
-        for (int j = 0; j < sy; j++)
-            for (int i = 0; i < sx; i++) {
-                const size_t index = (size_t)k * sx * sy + (size_t)j * sx + i;
-                const float v = real_array[index].x;
-                if (std::isnan(v)) {
-                    std::cout << "NaN found in field " << name << std::endl;
-                    std::exit(1);
-                }
-                int w = 0;
-                if (dim == 1) w = fprintf(fp, fmt.c_str(), i, v);
-                if (dim == 2) w = fprintf(fp, fmt.c_str(), i, j, v);
-                if (dim == 3) w = fprintf(fp, fmt.c_str(), i, j, k, v);
-                if (w < 0) {
-                    std::cout << "Error writing data to output file " << fileName << std::endl;
-                    std::exit(1);
-                }
-            }
-    fclose(fp);
+    // formatting and writing continue on the background writer
+    cupss_host::output_submit(std::move(job));
 }
 
 float field::getStepqx() { return stepqx; }
