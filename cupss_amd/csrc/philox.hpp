@@ -81,6 +81,17 @@ __device__ __forceinline__ float2 box_muller(uint32_t a, uint32_t b)
     return make_float2(r * cs.x, r * cs.y);
 }
 
+// Box-Muller with the hardware sine and cosine (argument in revolutions,
+// absolute error of order 1e-6): the spectral-space draw, whose checks are
+// statistical (its law), uses it; the real-space draw, which is compared
+// value by value with the oracle's stream, uses box_muller().
+__device__ __forceinline__ float2 box_muller_hw(uint32_t a, uint32_t b)
+{
+    const float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u01(a)));
+    const float w = u01(b);
+    return make_float2(r * __builtin_amdgcn_cosf(w), r * __builtin_amdgcn_sinf(w));
+}
+
 // The normals of cells `cell` and `cell+1` (cell even) of one stream.
 __device__ __forceinline__ float2 philox_normal_pair(uint32_t cell, uint32_t step_lo, uint32_t step_hi,
                                                      uint32_t stream, uint32_t key0, uint32_t key1)

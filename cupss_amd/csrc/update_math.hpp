@@ -302,7 +302,8 @@ __device__ __forceinline__ bool keep_mode(const Geom &g, int p, int kx, int ky, 
 // Draws are paired along y: bins (kx, j, kz) and (kx, j + hy, kz), with
 // hy = sy/2 (1 if sy = 1) and j < hy, share one Philox4x32-10 call with
 // counter (pair index, step_lo, step_hi, stream | 0x80000000); outputs 0,1
-// give the first bin's Box-Muller pair and outputs 2,3 the second's.
+// give the first bin's Box-Muller pair and outputs 2,3 the second's
+// (box_muller_hw).
 __device__ __forceinline__ int noise_hy(const Geom &g) { return g.sy >= 2 ? g.sy / 2 : 1; }
 
 __device__ __forceinline__ Philox4 noise_philox(const Geom &g, int kx, int j, int kz, uint32_t slo, uint32_t shi,
@@ -329,7 +330,7 @@ __device__ __forceinline__ float2 spectral_noise(const Geom &g, int kx, int ky, 
     }
     const int hy = noise_hy(g);
     const Philox4 o = noise_philox(g, kx, cy % hy, cz, slo, shi, stream, k0, k1);
-    const float2 z = (cy / hy) ? box_muller(o.x[2], o.x[3]) : box_muller(o.x[0], o.x[1]);
+    const float2 z = (cy / hy) ? box_muller_hw(o.x[2], o.x[3]) : box_muller_hw(o.x[0], o.x[1]);
     const float n = (float)g.n;
     if (real) return make_float2(__builtin_amdgcn_sqrtf(n) * z.x, 0.0f);
     const float s = __builtin_amdgcn_sqrtf(0.5f * n);
@@ -348,8 +349,8 @@ __device__ __forceinline__ void spectral_noise_pair(const Geom &g, int kx, int j
         return;
     }
     const Philox4 o = noise_philox(g, kx, j, kz, slo, shi, stream, k0, k1);
-    const float2 z0 = box_muller(o.x[0], o.x[1]);
-    const float2 z1 = box_muller(o.x[2], o.x[3]);
+    const float2 z0 = box_muller_hw(o.x[0], o.x[1]);
+    const float2 z1 = box_muller_hw(o.x[2], o.x[3]);
     const float s = __builtin_amdgcn_sqrtf(0.5f * (float)g.n);
     lo = make_float2(s * z0.x, s * z0.y);
     hi = make_float2(s * z1.x, s * z1.y);
