@@ -38,6 +38,31 @@ __device__ __forceinline__ float2 mul_i(float2 a)
     return DIR > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
 }
 
+// Global-memory access through a pointer read from a descriptor.  The
+// compiler cannot tell the address space of such a pointer and emits flat
+// instructions, which also count against the LDS wait counter (every LDS
+// wait of a transform would then wait for outstanding stores); through
+// these the accesses are global loads and stores.  (The host pass of the
+// compiler parses device functions too but has no such address space.)
+template <class T>
+__device__ __forceinline__ T gload(const T *p, long i)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ((const __attribute__((address_space(1))) T *)p)[i];
+#else
+    return p[i];
+#endif
+}
+template <class T>
+__device__ __forceinline__ void gstore(T *p, long i, const T &v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    ((__attribute__((address_space(1))) T *)p)[i] = v;
+#else
+    p[i] = v;
+#endif
+}
+
 // cos(2*pi*j/32), j = 0..31, and the matching sines, for the in-register
 // DFT twiddles (R <= 32).
 __device__ constexpr float kC32[32] = {

@@ -55,6 +55,10 @@ struct FRowCfg {
     }
 };
 
+// four floats as one 16-byte access (a native vector: HIP's float4 copies
+// through a generic pointer, which would make the access flat)
+typedef float vf4 __attribute__((ext_vector_type(4)));
+
 // rows y0 .. y0+LPB-1 of a column-major half spectrum into the tile (row l
 // at l*RS, entry k at k); columns k > kmax read as zero.  Pairs of rows are
 // moved with one 16-byte access.
@@ -66,14 +70,14 @@ __device__ __forceinline__ void tile_load(float2 *__restrict__ tile, const float
         constexpr int HP = LPB / 2;
         for (int i = threadIdx.x; i < (NC + 1) * HP; i += TH) {
             const int k = i / HP, yl = 2 * (i % HP);
-            float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (k <= kmax) w = *reinterpret_cast<const float4 *>(src + (long)k * sy + y0 + yl);
+            vf4 w = {0.f, 0.f, 0.f, 0.f};
+            if (k <= kmax) w = gload(reinterpret_cast<const vf4 *>(src + (long)k * sy + y0 + yl), 0);
             tile[yl * RS + k] = make_float2(w.x, w.y);
             tile[(yl + 1) * RS + k] = make_float2(w.z, w.w);
         }
     } else {
         for (int k = threadIdx.x; k <= NC; k += TH)
-            tile[k] = k <= kmax ? src[(long)k * sy + y0] : make_float2(0.f, 0.f);
+            tile[k] = k <= kmax ? gload(src, (long)k * sy + y0) : make_float2(0.f, 0.f);
     }
 }
 
@@ -85,10 +89,11 @@ __device__ __forceinline__ void tile_store(float2 *__restrict__ dst, const float
         for (int i = threadIdx.x; i < (NC + 1) * HP; i += TH) {
             const int k = i / HP, yl = 2 * (i % HP);
             const float2 a = tile[yl * RS + k], b = tile[(yl + 1) * RS + k];
-            *reinterpret_cast<float4 *>(dst + (long)k * sy + y0 + yl) = make_float4(a.x, a.y, b.x, b.y);
+            const vf4 w = {a.x, a.y, b.x, b.y};
+            gstore(reinterpret_cast<vf4 *>(dst + (long)k * sy + y0 + yl), 0, w);
         }
     } else {
-        for (int k = threadIdx.x; k <= NC; k += TH) dst[(long)k * sy + y0] = tile[k];
+        for (int k = threadIdx.x; k <= NC; k += TH) gstore(dst, (long)k * sy + y0, tile[k]);
     }
 }
 
@@ -249,7 +254,7 @@ __device__ __forceinline__ void target_sum(const FusedInst &a, const FusedTarget
     for (int gi = TG.g0; gi < TG.g1; gi++) {
         if (!(pre && gi == TG.g0)) {
 #pragma unroll
-            for (int e = 0; e < E; e++) xv[e] = active ? a.X[gi][col + t + e * T] : make_float2(0.f, 0.f);
+            for (int e = 0; e < E; e++) xv[e] = active ? gload(a.X[gi], col + t + e * T) : make_float2(0.f, 0.f);
         }
         fft_line<N, E, -1, N>(xv, L, t, tws);
         const TermRefDev &GM = a.gmul[gi];
@@ -277,7 +282,7 @@ __device__ __forceinline__ void target_sum(const FusedInst &a, const FusedTarget
         } else {
             const float2 *__restrict__ src = a.cs[TR.idx][parity];
 #pragma unroll
-            for (int e = 0; e < E; e++) X[e] = active ? src[col + t + e * T] : make_float2(0.f, 0.f);
+            for (int e = 0; e < E; e++) X[e] = active ? gload(src, col + t + e * T) : make_float2(0.f, 0.f);
         }
 #pragma unroll
         for (int e = 0; e < E; e++) acc[e] = cadd(acc[e], apply_mult(m[e], TR.mult_i, X[e]));
@@ -298,7 +303,7 @@ __device__ __forceinline__ void z_out(const FusedInst &a, int f, const Geom &g, 
     fft_line<N, E, +1, N>(v, L, t, tws);
     if (active && cb.kx <= a.kmax[f]) {
 #pragma unroll
-        for (int e = 0; e < E; e++) a.Z[f][col + t + e * T] = v[e];
+        for (int e = 0; e < E; e++) gstore(a.Z[f], col + t + e * T, v[e]);
     }
     __syncthreads();
 }
@@ -341,13 +346,12 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
     const bool pre = upd && a.tgt[0].g1 > a.tgt[0].g0;  // phi has product groups: load the first early
     const float2 *__restrict__ sin = a.spec[parity];
 
-    float2 cur[E], xv[E], eta[E];
+    float2 cur[E], xv[E];
 #pragma unroll
     for (int e = 0; e < E; e++) {
         const long i = col + t + e * T;
-        cur[e] = active ? sin[i] : make_float2(0.f, 0.f);
-        xv[e] = (active && pre) ? a.X[a.tgt[0].g0][i] : make_float2(0.f, 0.f);
-        eta[e] = (active && upd && a.noise_mode == NOISE_REAL) ? a.eta[i] : make_float2(0.f, 0.f);
+        cur[e] = active ? gload(sin, i) : make_float2(0.f, 0.f);
+        xv[e] = (active && pre) ? gload(a.X[a.tgt[0].g0], i) : make_float2(0.f, 0.f);
     }
     load_twiddles<N>(tws, sets);
     __syncthreads();
@@ -360,7 +364,14 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
         // ---- phi (setDynamic_k, src/field_kernels.cu:199-227)
         float2 acc[E];
         target_sum<N, E, T>(a, a.tgt[0], cb, cur, cur, xv, pre, parity, active, col, L, t, tws, acc);
+        float2 eta[E];
+#pragma unroll
+        for (int e = 0; e < E; e++) eta[e] = make_float2(0.f, 0.f);
         if (a.noise_mode == NOISE_REAL) {
+            // loaded here rather than with phi: fewer registers live across
+            // the transforms of the terms
+#pragma unroll
+            for (int e = 0; e < E; e++) eta[e] = active ? gload(a.eta, col + t + e * T) : make_float2(0.f, 0.f);
             fft_line<N, E, -1, N>(eta, L, t, tws);
             __syncthreads();
         } else if (a.noise_mode == NOISE_SPECTRAL) {
@@ -391,7 +402,7 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
         if (active) {
             float2 *__restrict__ sout = a.spec[parity ^ 1];
 #pragma unroll
-            for (int e = 0; e < E; e++) sout[col + t + e * T] = nw[e];
+            for (int e = 0; e < E; e++) gstore(sout, col + t + e * T, nw[e]);
         }
         // ---- stored constraints (setNotDynamic_k, src/field_kernels.cu:130-197):
         // their values of step m+1, from the pre-step state of step m+1
@@ -432,7 +443,7 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
             if (active) {
                 float2 *__restrict__ cout = a.cs[ci][parity ^ 1];
 #pragma unroll
-                for (int e = 0; e < E; e++) cout[col + t + e * T] = v[e];
+                for (int e = 0; e < E; e++) gstore(cout, col + t + e * T, v[e]);
             }
             if (TG.z >= 0) z_out<N, E, T>(a, TG.z, g, cb, v, false, blk.line, active, col, L, t, tws);
         }
@@ -445,7 +456,7 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
             if (z < 0) continue;
             float2 v[E];
 #pragma unroll
-            for (int e = 0; e < E; e++) v[e] = active ? a.cs[ci][0][col + t + e * T] : make_float2(0.f, 0.f);
+            for (int e = 0; e < E; e++) v[e] = active ? gload(a.cs[ci][0], col + t + e * T) : make_float2(0.f, 0.f);
             z_out<N, E, T>(a, z, g, cb, v, a.zero[z] != 0, blk.line, active, col, L, t, tws);
         }
     }
@@ -489,12 +500,12 @@ __device__ __forceinline__ void tile_to_cm(float2 (&s)[FUSED_TILE][FUSED_TILE + 
     constexpr int S = FUSED_TILE;
     for (int q = threadIdx.x; q < S * S; q += blockDim.x) {
         const int i = q % S, j = q / S;
-        if (x0 + i < g.nc1 && y0 + j < g.sy) s[j][i] = src[(long)(y0 + j) * g.nc1 + x0 + i];
+        if (x0 + i < g.nc1 && y0 + j < g.sy) s[j][i] = gload(src, (long)(y0 + j) * g.nc1 + x0 + i);
     }
     __syncthreads();
     for (int q = threadIdx.x; q < S * S; q += blockDim.x) {
         const int j = q % S, i = q / S;
-        if (x0 + i < g.nc1 && y0 + j < g.sy) dst[(long)(x0 + i) * g.sy + y0 + j] = s[j][i];
+        if (x0 + i < g.nc1 && y0 + j < g.sy) gstore(dst, (long)(x0 + i) * g.sy + y0 + j, s[j][i]);
     }
     __syncthreads();
 }
@@ -509,7 +520,7 @@ __device__ __forceinline__ void tile_to_nat(float2 (&s)[FUSED_TILE][FUSED_TILE +
     const Geom &g = a.g;
     for (int q = threadIdx.x; q < S * S; q += blockDim.x) {
         const int j = q % S, i = q / S;
-        if (x0 + i < g.nc1 && y0 + j < g.sy) s[j][i] = src[(long)(x0 + i) * g.sy + y0 + j];
+        if (x0 + i < g.nc1 && y0 + j < g.sy) s[j][i] = gload(src, (long)(x0 + i) * g.sy + y0 + j);
     }
     __syncthreads();
     for (int q = threadIdx.x; q < S * S; q += blockDim.x) {
@@ -518,10 +529,10 @@ __device__ __forceinline__ void tile_to_nat(float2 (&s)[FUSED_TILE][FUSED_TILE +
         if (kx < g.nc1 && ky < g.sy) {
             const long h = (long)ky * g.nc1 + kx;
             if (M.npres == 0) {
-                dst[h] = s[j][i];
+                gstore(dst, h, s[j][i]);
             } else {
                 const Bin b = make_bin(g, kx, ky, 0);
-                dst[h] = apply_mult(multiplier(a.pool, M.pres0, M.npres, M.mult_i, b), M.mult_i, s[j][i]);
+                gstore(dst, h, apply_mult(multiplier(a.pool, M.pres0, M.npres, M.mult_i, b), M.mult_i, s[j][i]));
             }
         }
     }
@@ -613,8 +624,9 @@ __global__ __launch_bounds__(FUSED_ELEM_BLOCK) void k_felem(const FusedInst *__r
             if (P.invq != 0) th *= ipow(b[w].iqd, P.invq);
             D[w] -= u.dt * th;
         }
+        D[w] = __builtin_amdgcn_rcpf(D[w]);  // 1/D(q): hardware reciprocal, once per bin
         pn[w] = noisy ? noise_amplitude(u, b[w]) : 0.f;
-        cur[w] = a.spec[parity][b[w].h];
+        cur[w] = gload(a.spec[parity], b[w].h);
     }
     const uint64_t s0 = a.step_base + step_rel;
     for (int n = 0; n < nsteps; n++) {
@@ -623,8 +635,8 @@ __global__ __launch_bounds__(FUSED_ELEM_BLOCK) void k_felem(const FusedInst *__r
                 const TermRefDev &M = a.dmul[d];
 #pragma unroll
                 for (int w = 0; w < 2; w++)
-                    a.dout[d][b[w].h] =
-                        apply_mult(multiplier(a.pool, M.pres0, M.npres, M.mult_i, b[w]), M.mult_i, cur[w]);
+                    gstore(a.dout[d], b[w].h,
+                           apply_mult(multiplier(a.pool, M.pres0, M.npres, M.mult_i, b[w]), M.mult_i, cur[w]));
             }
         }
         float2 eta[2] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
@@ -644,14 +656,14 @@ __global__ __launch_bounds__(FUSED_ELEM_BLOCK) void k_felem(const FusedInst *__r
                 s.y += pn[w] * eta[w].y;
             }
             if (u.nimp > 0) {
-                s.x /= D[w];
-                s.y /= D[w];
+                s.x *= D[w];
+                s.y *= D[w];
             }
             cur[w] = s;
         }
     }
 #pragma unroll
-    for (int w = 0; w < 2; w++) a.spec[parity ^ (nsteps & 1)][b[w].h] = cur[w];
+    for (int w = 0; w < 2; w++) gstore(a.spec[parity ^ (nsteps & 1)], b[w].h, cur[w]);
 }
 
 // ------------------------------------------------------------ dispatch
