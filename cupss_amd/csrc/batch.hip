@@ -110,6 +110,11 @@ struct FusedBatch {
     long steps_run = 0;
     std::vector<uint64_t> gens;   // system generations the tables were built for
 
+    // 3D path (the batch of one of a 3D system): h[0] is the descriptor on
+    // the device, twd3 the z-line tile width
+    bool d3 = false;
+    int twd3 = 0;
+
     ~FusedBatch() { release(); }
 
     void drop_graph()
@@ -212,8 +217,9 @@ static int pool_term(FusedInst &a, int &np, const std::vector<cupss_pres> &pres,
     return CUPSS_OK;
 }
 
-// device descriptor of system s (its fused plan must be ok)
-static int describe(cupss_system s, FusedInst &a)
+// device descriptor of system s (its fused plan must be ok); natural: the
+// 3D path, which needs no column-major spectra
+static int describe(cupss_system s, FusedInst &a, bool natural = false)
 {
     const FusedPlan &P = s->plan;
     const Geom &g = s->g;
@@ -328,7 +334,7 @@ static int describe(cupss_system s, FusedInst &a)
             a.tgt[1 + c].z = (int)i;
         }
     }
-    if (!is_elementwise(a)) {
+    if (!natural && !is_elementwise(a)) {
         if (int r = alloc_spec(&s->ft[0], g)) return r;
         if (int r = alloc_spec(&s->ft[1], g)) return r;
     }
@@ -848,6 +854,109 @@ static int run(FusedBatch &B, int nsteps)
     return settle(B, nsteps);
 }
 
+// The fused step of a 3D system (fused.hip: k_f3rows, k_f3z; the y lines
+// by lines_y), natural layout, no transposes.  A run is
+//   prologue:  z pass in mode 0 and IFFT_y: W_f, Z_f of the current state;
+//   steps:     rows, FFT_y of every group, z pass (update; W_f of the new
+//              state except in the run's last step), IFFT_y of every real
+//              field;
+//   end:       the derived fields of the last step's starting phi.
+// The descriptor keeps both spectrum buffers of phi in a fixed order (the
+// launches pass the parity) and is uploaded again only when it changes.
+static int run3d(FusedBatch &B, cupss_system s, int nsteps)
+{
+    const FusedPlan &P = s->plan;
+    FieldRec &F = s->f[P.phi];
+    const Geom &g = s->g;
+    FusedInst a;
+    if (int r = describe(s, a, true)) return r;
+    for (int i = 0; i < a.nreal; i++) {
+        if (!s->fw[i]) {
+            if (int r = alloc_spec(&s->fw[i], g)) return r;
+            HIP_TRY(hipMemsetAsync(s->fw[i], 0, sizeof(float2) * g.nh, s->stream));  // lines off the mask stay zero
+        }
+        a.W[i] = s->fw[i];
+    }
+    a.spec[0] = F.spec[0];
+    a.spec[1] = F.spec[1];
+    a.nat = nullptr;
+    a.u.cur = nullptr;  // per-run buffers of the general update, unused here (and they
+    a.u.out = nullptr;  // would make every run's descriptor differ)
+    for (int k = 0; k < a.nreal; k++) a.zero[k] = (P.real[k] == P.phi && F.dstate == 0) ? 1 : 0;
+    for (int d = 0; d < a.nder; d++) {
+        FieldRec &D = s->f[P.derived[d]];
+        a.dout[d] = D.spec[D.cur];
+    }
+    bool same = B.d3 && B.h.size() == 1;
+    if (same) {
+        const uint64_t base = B.h[0].step_base;
+        a.step_base = base;
+        same = std::memcmp(&a, &B.h[0], sizeof(FusedInst)) == 0 && s->step >= base &&
+               s->step - base + (uint64_t)nsteps < (1ull << 31);
+    }
+    if (!same) {
+        a.step_base = s->step;
+        HIP_TRY(hipStreamSynchronize(s->stream));  // the staging buffer may still feed an earlier copy
+        if (B.dcap < 1) {
+            hipFree(B.d);
+            hipHostFree(B.pinned);
+            B.d = nullptr;
+            B.pinned = nullptr;
+            if (hipMalloc(&B.d, sizeof(FusedInst)) != hipSuccess ||
+                hipHostMalloc(&B.pinned, sizeof(FusedInst)) != hipSuccess)
+                return fail(CUPSS_ERR_NOMEM, "descriptor allocation failed");
+            B.dcap = 1;
+        }
+        std::memcpy(B.pinned, &a, sizeof(FusedInst));
+        HIP_TRY(hipMemcpyAsync(B.d, B.pinned, sizeof(FusedInst), hipMemcpyHostToDevice, s->stream));
+        B.h.assign(1, a);
+        B.d3 = true;
+    }
+    if (!B.twd3) {
+        const std::vector<int> v = fused3d_z_shapes(g.sz);
+        if (v.empty()) return fail(CUPSS_ERR_UNSUPPORTED, "no z-line tile for this size");
+        B.twd3 = v[0];
+        for (int w : v)
+            if (w <= 8) B.twd3 = w;
+    }
+    if (!twiddle_sets(s->stream) || !twiddles(s->stream)) return fail(CUPSS_ERR_NOMEM, "twiddle tables");
+    const uint32_t rel0 = (uint32_t)(s->step - B.h[0].step_base);
+    const int par0 = F.cur, maxr = a.nreal > 3 ? FUSED_MAX_REAL : 3;
+    const long nrows = (long)g.sy * g.sz;
+    hipStream_t st = s->stream;
+    if (a.nreal > 0) {
+        HIP_TRY(fused3d_z_launch(g.sz, B.twd3, B.d, g.nc1, g.sy, par0, rel0, 0, st));
+        for (int f = 0; f < a.nreal; f++) HIP_TRY(lines_y(g, +1, s->fw[f], s->fz[f], a.kmax[f] + 1, st));
+    }
+    if (B.profile) {
+        B.pclass = -2;
+        B.pcount = 0;
+        HIP_TRY(fused_ts_reset_launch(B.pts, FusedBatch::PROF_SLOTS, st));
+    }
+    for (int i = 0; i < nsteps; i++) {
+        const bool last = i == nsteps - 1;
+        if (a.ngroups > 0) {
+            HIP_TRY(fused3d_rows_launch(g.nc, maxr, B.d, nrows, st));
+            for (int gi = 0; gi < a.ngroups; gi++) HIP_TRY(lines_y(g, -1, s->fx[gi], s->fx[gi], g.nc1, st));
+        }
+        unsigned long long *ts = nullptr;
+        if (B.profile && B.pcount < FusedBatch::PROF_SLOTS)
+            ts = B.pts + (size_t)(B.pcount++) * FUSED_TS_SLOTS * FUSED_TS_STRIDE;
+        HIP_TRY(fused3d_z_launch(g.sz, B.twd3, B.d, g.nc1, g.sy, par0 ^ (i & 1), rel0 + (uint32_t)i, last ? 2 : 1, st,
+                                 ts));
+        if (!last)
+            for (int f = 0; f < a.nreal; f++) HIP_TRY(lines_y(g, +1, s->fw[f], s->fz[f], a.kmax[f] + 1, st));
+    }
+    if (a.nder > 0) HIP_TRY(fused3d_derived_launch(B.d, g.nh, par0 ^ ((nsteps - 1) & 1), st));
+    F.cur = par0 ^ (nsteps & 1);
+    s->step += (uint64_t)nsteps;
+    s->fused_steps += nsteps;
+    if (F.order > 0) F.dstate = 2;
+    for (int dfi : P.derived)
+        if (s->f[dfi].order > 0) s->f[dfi].dstate = 2;
+    return CUPSS_OK;
+}
+
 int fused_steps(cupss_system s, int nsteps)
 {
     FusedBatch *B = static_cast<FusedBatch *>(s->batch);
@@ -856,6 +965,10 @@ int fused_steps(cupss_system s, int nsteps)
         B->sys.push_back(s);
         B->stream = s->stream;
         s->batch = B;
+    }
+    if (s->g.sz > 1) {
+        B->stream = s->stream;
+        return nsteps > 0 ? run3d(*B, s, nsteps) : CUPSS_OK;
     }
     if (B->stream != s->stream) {
         B->stream = s->stream;
@@ -924,7 +1037,7 @@ extern "C" int cupss_batch_add(cupss_batch b, cupss_system s)
 {
     if (!b || !s) return fail(CUPSS_ERR_INVALID, "null batch or system");
     if (!s->prepared) return fail(CUPSS_ERR_STATE, "prepare the system before adding it to a batch");
-    if (!s->plan.ok) return fail(CUPSS_ERR_UNSUPPORTED, "system does not qualify for the fused batched step");
+    if (!s->plan.ok || s->g.sz > 1) return fail(CUPSS_ERR_UNSUPPORTED, "system does not qualify for the fused batched step");
     for (auto &F : s->f)
         if (F.real_hook || F.spec_hook) return fail(CUPSS_ERR_UNSUPPORTED, "systems with callbacks cannot be batched");
     if (s->f[s->plan.phi].dstate == 3) {
@@ -991,7 +1104,7 @@ static int set_profile(FusedBatch &B, int on)
 
 static int kernel_time(FusedBatch &B, float *avg_ms, double *points, char *name, int len)
 {
-    if (!B.profile || B.pclass < 0 || B.pcount == 0) return fail(CUPSS_ERR_STATE, "no timed launches yet");
+    if (!B.profile || B.pclass == -1 || B.pcount == 0) return fail(CUPSS_ERR_STATE, "no timed launches yet");
     HIP_TRY(hipStreamSynchronize(B.stream));
     const size_t per = (size_t)FUSED_TS_SLOTS * FUSED_TS_STRIDE;
     std::vector<unsigned long long> raw(per * B.pcount), h(2 * B.pcount);
@@ -1016,10 +1129,15 @@ static int kernel_time(FusedBatch &B, float *avg_ms, double *points, char *name,
         if (h[2 * j + 1] < h[2 * j]) return fail(CUPSS_ERR_STATE, "timed launch did not record");
         sum += (double)(h[2 * j + 1] - h[2 * j]) / (double)khz;
     }
+    if (avg_ms) *avg_ms = (float)(sum / B.pcount);
+    if (B.pclass == -2) {  // 3D: the z pass of the one system
+        if (points) *points = (double)B.h[0].g.n;
+        if (name && len > 0) std::snprintf(name, (size_t)len, "k_f3z<%d, %d>", B.h[0].g.sz, B.twd3);
+        return CUPSS_OK;
+    }
     const auto &c = B.classes[B.pclass];
     double pts = 0;
     for (int i : c.insts) pts += (double)B.h[i].g.n;
-    if (avg_ms) *avg_ms = (float)(sum / B.pcount);
     if (points) *points = pts;
     if (name && len > 0) std::snprintf(name, (size_t)len, "k_fcols<%d, %d>", c.ny, c.twd);
     return CUPSS_OK;

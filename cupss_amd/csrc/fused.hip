@@ -666,6 +666,328 @@ __global__ __launch_bounds__(FUSED_ELEM_BLOCK) void k_felem(const FusedInst *__r
     for (int w = 0; w < 2; w++) gstore(a.spec[parity ^ (nsteps & 1)], b[w].h, cur[w]);
 }
 
+// ------------------------------------------------------------ 3D systems
+//
+// A 3D system runs alone (insts[0]) with every buffer in the natural half-
+// spectrum layout, element (kx, ky, kz) at (kz*sy + ky)*nc1 + kx, so no
+// transposes are needed: rows are contiguous, and lines along y and z are
+// read TWD adjacent kx at a time.  Per step:
+//   rows  (k_f3rows): Z_f rows -C2R_x-> real fields -> products -R2C_x-> X_g
+//   y     (lines_y):  X_g = FFT_y(X_g)
+//   z     (k_f3z):    X_g -FFT_z-> terms; update of phi (setDynamic_k);
+//                     mask_f * L_f * phi_new -IFFT_z-> W_f
+//   y     (lines_y):  Z_f = IFFT_y(W_f), lines kx <= kmax_f
+// W_f is written only on the (kx, ky) lines the dealias mask of f keeps; the
+// others stay zero from allocation.  The systems have no stored constraint
+// fields and no real-space noise (the host checks).
+
+template <int NC, int LPB, int MAXR>
+__global__ __launch_bounds__((FRowCfg<NC, LPB, MAXR>::THREADS)) void k_f3rows(const FusedInst *__restrict__ insts,
+                                                                           const float2 *__restrict__ sets)
+{
+    using C = FRowCfg<NC, LPB, MAXR>;
+    constexpr int E = C::E, T = C::T, RS = C::RS;
+    __shared__ float2 tws[2 * NC];
+    __shared__ float2 tile[LPB * RS];
+    const FusedInst &a = insts[0];
+    const int nc1 = a.g.nc1;
+    const long nrows = (long)a.g.sy * a.g.sz;
+    const int l = threadIdx.x / T, t = threadIdx.x % T;
+    const long row = (long)blockIdx.x * LPB + l;
+    const bool active = row < nrows;
+    float2 *L = tile + l * RS;
+    const int nreal = a.nreal;
+    load_twiddles<2 * NC>(tws, sets);
+    __syncthreads();
+
+    float2 r[MAXR][E];
+#pragma unroll
+    for (int f = 0; f < MAXR; f++) {
+        if (f < nreal) {
+            const float2 *__restrict__ z = a.Z[f] + row * nc1;
+            const int km = a.kmax[f];
+            float2 xa[E], ya[E], v[E];
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int k = t + e * T;
+                xa[e] = (active && k <= km) ? gload(z, k) : make_float2(0.f, 0.f);
+                ya[e] = (active && NC - k <= km) ? gload(z, NC - k) : make_float2(0.f, 0.f);
+            }
+            c2r_prep<NC, E, T>(xa, ya, t, v, tws);
+            fft_line<NC, E, +1, 2 * NC>(v, L, t, tws);
+#pragma unroll
+            for (int e = 0; e < E; e++) r[f][e] = cscale(v[e], a.scale);
+            __syncthreads();
+        }
+    }
+    for (int gi = 0; gi < a.ngroups; gi++) {
+        float2 v[E];
+#pragma unroll
+        for (int e = 0; e < E; e++) v[e] = make_float2(0.f, 0.f);
+        for (int pi = a.pg0[gi]; pi < a.pg0[gi + 1]; pi++) {
+            float2 p[E];
+            const float c0 = a.coef[pi];
+#pragma unroll
+            for (int e = 0; e < E; e++) p[e] = make_float2(c0, c0);
+#pragma unroll
+            for (int f = 0; f < MAXR; f++)
+                for (int c = 0; c < a.cnt[pi][f]; c++) {
+#pragma unroll
+                    for (int e = 0; e < E; e++) {
+                        p[e].x *= r[f][e].x;
+                        p[e].y *= r[f][e].y;
+                    }
+                }
+#pragma unroll
+            for (int e = 0; e < E; e++) v[e] = cadd(v[e], p[e]);
+        }
+        r2c_tile<NC, E, T>(v, L, t, tws);
+        if (active) {
+            float2 *__restrict__ x = a.X[gi] + row * nc1;
+            for (int k = t; k <= NC; k += T) gstore(x, k, L[k]);
+        }
+        __syncthreads();
+    }
+}
+
+// q of the E bins (kx, ky, t + e*T) that one lane of a z-line kernel holds
+template <int E>
+struct LineBins {
+    float qx, qy;
+    bool nyx, nyy;
+    bool axis;              // kx = ky = 0 (the reference's i = j = 0 line, Q8)
+    float qz[E], q2[E];
+    bool nyz[E], zero[E];   // Nyquist plane; the q = 0 bin
+};
+
+template <int E, int T>
+__device__ __forceinline__ void line_bins(const Geom &g, int kx, int ky, int t, LineBins<E> &lb)
+{
+    lb.qx = (float)(kx < (g.sx + 1) / 2 ? kx : kx - g.sx) * g.sqx;
+    lb.qy = (float)(ky < (g.sy + 1) / 2 ? ky : ky - g.sy) * g.sqy;
+    lb.nyx = g.sx > 1 && kx == g.sx / 2;
+    lb.nyy = g.sy > 1 && ky == g.sy / 2;
+    lb.axis = kx == 0 && ky == 0;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int kz = t + e * T;
+        lb.qz[e] = (float)(kz < (g.sz + 1) / 2 ? kz : kz - g.sz) * g.sqz;
+        lb.q2[e] = lb.qx * lb.qx + lb.qy * lb.qy + lb.qz[e] * lb.qz[e];
+        lb.nyz[e] = g.sz > 1 && kz == g.sz / 2;
+        lb.zero[e] = lb.axis && kz == 0;
+    }
+}
+
+// multiplier() for the E bins of a z line (same operation order)
+template <int E>
+__device__ __forceinline__ void multiplier_line(const PresDev *__restrict__ pres, int p0, int np, int mult_i,
+                                                const LineBins<E> &lb, float (&m)[E])
+{
+#pragma unroll
+    for (int e = 0; e < E; e++) m[e] = 0.0f;
+    for (int p = p0; p < p0 + np; p++) {
+        const PresDev P = pres[p];
+        const int parl = mult_i + (lb.nyx ? 0 : P.a) + (lb.nyy ? 0 : P.b);
+        const float qxa = ipow(lb.qx, P.a), qyb = ipow(lb.qy, P.b);
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int par = parl + (lb.nyz[e] ? 0 : P.cz);
+            float v = (par & 1) ? 0.0f : P.c;
+            if (P.q2n > 0) v *= ipow(lb.q2[e], P.q2n);
+            if (P.a > 0) v *= qxa;
+            if (P.b > 0) v *= qyb;
+            if (P.cz > 0) v *= ipow(lb.qz[e], P.cz);
+            if (P.invq > 0) v *= ipow(lb.zero[e] ? 0.0f : __builtin_amdgcn_rsqf(lb.q2[e]), P.invq);
+            m[e] += v;
+        }
+    }
+}
+
+template <int N, int TWD>
+struct F3zCfg {
+    static constexpr int E = N >= 8 ? 8 : N;
+    static constexpr int T = N / E;
+    static constexpr int THREADS = TWD * T;
+    static constexpr int LL = lds_len(N);
+    static constexpr bool valid() { return THREADS >= 64 && THREADS <= 512; }
+};
+
+// Lines along z of TWD adjacent kx at one ky (thread = t*TWD + c: a row of
+// TWD values is one contiguous access).  Workgroups are placed XCD-aware.
+template <int N, int TWD>
+__global__ __launch_bounds__((F3zCfg<N, TWD>::THREADS)) void k_f3z(const FusedInst *__restrict__ insts, int parity,
+                                                                 uint32_t step_rel, int mode,
+                                                                 const float2 *__restrict__ sets,
+                                                                 unsigned long long *__restrict__ ts)
+{
+    using C = F3zCfg<N, TWD>;
+    constexpr int E = C::E, T = C::T;
+    __shared__ float2 tws[N];
+    __shared__ float2 lds[TWD * C::LL];
+    const FusedInst &a = insts[0];
+    const Geom &g = a.g;
+    const int ntx = (g.nc1 + TWD - 1) / TWD;
+    const long item = xcd_item(blockIdx.x, gridDim.x);
+    if (item >= (long)ntx * g.sy) return;  // whole workgroup idle
+    unsigned long long *slot = ts ? ts + (size_t)(blockIdx.x % FUSED_TS_SLOTS) * FUSED_TS_STRIDE : nullptr;
+    if (slot && threadIdx.x == 0) atomicMin(&slot[0], (unsigned long long)wall_clock64());
+    const int kx0 = (int)(item % ntx) * TWD, ky = (int)(item / ntx);
+    const int c = threadIdx.x % TWD, t = threadIdx.x / TWD;
+    const int kx = kx0 + c;
+    const bool active = kx < g.nc1;
+    float2 *L = lds + c * C::LL;
+    const long plane = (long)g.nc1 * g.sy;
+    const long base = (long)ky * g.nc1 + kx;  // + kz * plane
+    const bool upd = mode != 0;
+    const FusedTarget &TG = a.tgt[0];
+    const float2 *__restrict__ sin = a.spec[parity];
+
+    float2 cur[E], xv[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const long i = base + (long)(t + e * T) * plane;
+        cur[e] = active ? gload(sin, i) : make_float2(0.f, 0.f);
+        xv[e] = (active && upd && TG.g1 > TG.g0) ? gload(a.X[TG.g0], i) : make_float2(0.f, 0.f);
+    }
+    load_twiddles<N>(tws, sets);
+    __syncthreads();
+
+    LineBins<E> lb;
+    line_bins<E, T>(g, kx, ky, t, lb);
+    float2 nw[E];
+    if (upd) {
+        float m[E];
+        float2 acc[E];
+#pragma unroll
+        for (int e = 0; e < E; e++) acc[e] = make_float2(0.f, 0.f);
+        // product groups: FFT_z of X_g times the group multiplier
+        for (int gi = TG.g0; gi < TG.g1; gi++) {
+            if (gi != TG.g0) {
+#pragma unroll
+                for (int e = 0; e < E; e++)
+                    xv[e] = active ? gload(a.X[gi], base + (long)(t + e * T) * plane) : make_float2(0.f, 0.f);
+            }
+            fft_line<N, E, -1, N>(xv, L, t, tws);
+            const TermRefDev &GM = a.gmul[gi];
+            multiplier_line<E>(a.pool, GM.pres0, GM.npres, GM.mult_i, lb, m);
+#pragma unroll
+            for (int e = 0; e < E; e++) acc[e] = cadd(acc[e], apply_mult(m[e], GM.mult_i, xv[e]));
+            __syncthreads();
+        }
+        // single-field and constant terms (phi or a derived field of phi)
+        for (int ti = 0; ti < TG.nterms; ti++) {
+            const FusedTerm &TR = TG.term[ti];
+            multiplier_line<E>(a.pool, TR.pres0, TR.npres, TR.mult_i, lb, m);
+            float2 X[E];
+            if (TR.src == SRC_CONST) {
+#pragma unroll
+                for (int e = 0; e < E; e++) X[e] = lb.zero[e] ? make_float2((float)g.n, 0.f) : make_float2(0.f, 0.f);
+            } else if (TR.src == SRC_DERIVED) {
+                const TermRefDev &DM = a.dmul[TR.idx];
+                float dm[E];
+                multiplier_line<E>(a.pool, DM.pres0, DM.npres, DM.mult_i, lb, dm);
+#pragma unroll
+                for (int e = 0; e < E; e++) X[e] = apply_mult(dm[e], DM.mult_i, cur[e]);
+            } else {
+#pragma unroll
+                for (int e = 0; e < E; e++) X[e] = cur[e];
+            }
+#pragma unroll
+            for (int e = 0; e < E; e++) acc[e] = cadd(acc[e], apply_mult(m[e], TR.mult_i, X[e]));
+        }
+        const UpdateArgs &u = a.u;
+        const uint64_t step = a.step_base + step_rel;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            float2 s = cur[e];
+            s.x += u.dt * acc[e].x;
+            s.y += u.dt * acc[e].y;
+            if (u.noisy && a.noise_mode == NOISE_SPECTRAL) {
+                const float2 eta = active ? spectral_noise(g, kx, ky, t + e * T, (uint32_t)step, (uint32_t)(step >> 32),
+                                                           a.stream, a.key0, a.key1)
+                                          : make_float2(0.f, 0.f);
+                float pn = u.noise_base;
+                if (u.noise_q2n != 0) pn *= sqrtf(ipow(lb.q2[e], u.noise_q2n));
+                if (u.noise_invq != 0)
+                    pn *= sqrtf(ipow(lb.zero[e] ? 0.0f : __builtin_amdgcn_rsqf(lb.q2[e]), u.noise_invq));
+                s.x += pn * eta.x;
+                s.y += pn * eta.y;
+            }
+            if (u.nimp > 0) {
+                float D = 1.0f;
+                for (int i = 0; i < u.nimp; i++) {
+                    const PresDev P = u.imp[i];
+                    float th = P.c;
+                    if (P.q2n != 0) th *= ipow(lb.q2[e], P.q2n);
+                    if (P.invq != 0) th *= ipow(lb.axis ? 0.0f : __builtin_amdgcn_rsqf(lb.q2[e]), P.invq);
+                    D -= u.dt * th;
+                }
+                const float rD = __builtin_amdgcn_rcpf(D);
+                s.x *= rD;
+                s.y *= rD;
+            }
+            nw[e] = s;
+        }
+        if (active) {
+            float2 *__restrict__ sout = a.spec[parity ^ 1];
+#pragma unroll
+            for (int e = 0; e < E; e++) gstore(sout, base + (long)(t + e * T) * plane, nw[e]);
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < E; e++) nw[e] = cur[e];
+    }
+    if (mode != 2) {
+        // W_f = IFFT_z(mask_f * L_f * phi) on the lines the mask keeps
+        const int ny = ky > g.sy / 2 ? ky - g.sy : ky;
+        for (int f = 0; f < a.nreal; f++) {
+            if (kx0 > a.kmax[f] || abs(ny) > a.kymax[f]) continue;  // uniform over the workgroup
+            const int lz = a.p[f] > 0 ? g.sz / (a.p[f] + 1) : g.sz;
+            const bool zero = !upd && a.zero[f] != 0;
+            float2 v[E];
+            const TermRefDev &LM = a.lmul[f];
+            float lm[E];
+            if (LM.npres > 0) multiplier_line<E>(a.pool, LM.pres0, LM.npres, LM.mult_i, lb, lm);
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int kz = t + e * T;
+                const int nz = kz > g.sz / 2 ? kz - g.sz : kz;
+                const bool keep = !zero && kx <= a.kmax[f] && abs(nz) <= lz;
+                const float2 w = LM.npres > 0 ? apply_mult(lm[e], LM.mult_i, nw[e]) : nw[e];
+                v[e] = keep ? w : make_float2(0.f, 0.f);
+            }
+            fft_line<N, E, +1, N>(v, L, t, tws);
+            if (active && kx <= a.kmax[f]) {
+                float2 *__restrict__ w = a.W[f];
+#pragma unroll
+                for (int e = 0; e < E; e++) gstore(w, base + (long)(t + e * T) * plane, v[e]);
+            }
+            __syncthreads();
+        }
+    }
+    if (slot) {
+        __syncthreads();
+        if (threadIdx.x == 0) atomicMax(&slot[1], (unsigned long long)wall_clock64());
+    }
+}
+
+// derived fields psi_d = M_d phi from phi's spectrum [parity] (natural layout)
+__global__ __launch_bounds__(256) void k_f3der(const FusedInst *__restrict__ insts, int parity)
+{
+    const FusedInst &a = insts[0];
+    const Geom &g = a.g;
+    const long h = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= g.nh) return;
+    const int kx = (int)(h % g.nc1);
+    const long r = h / g.nc1;
+    const Bin b = make_bin(g, kx, (int)(r % g.sy), (int)(r / g.sy));
+    const float2 X = gload(a.spec[parity], h);
+    for (int d = 0; d < a.nder; d++) {
+        const TermRefDev &M = a.dmul[d];
+        gstore(a.dout[d], h, apply_mult(multiplier(a.pool, M.pres0, M.npres, M.mult_i, b), M.mult_i, X));
+    }
+}
+
 // ------------------------------------------------------------ dispatch
 
 #define FUSED_ROW_CASES(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048)
@@ -673,7 +995,11 @@ __global__ __launch_bounds__(FUSED_ELEM_BLOCK) void k_felem(const FusedInst *__r
 #define FUSED_LPB(X) X(1) X(2) X(4) X(8) X(16)
 #define FUSED_TWD(X) X(1) X(2) X(4) X(8) X(16)
 
-bool fused_supported(int sx, int sy, int sz) { return sz == 1 && sx >= 16 && sx <= 4096 && sy >= 8 && sy <= 4096; }
+bool fused_supported(int sx, int sy, int sz)
+{
+    if (sz > 1) return sz >= 8 && sz <= 1024 && sx >= 16 && sx <= 2048 && sy >= 8 && sy <= 4096;  // 3D
+    return sx >= 16 && sx <= 4096 && sy >= 8 && sy <= 4096;
+}
 
 // wide row kernels (more than 3 real fields) for the sizes such systems use
 #define FUSED_WIDE_NC(NC) ((NC) >= 32 && (NC) <= 1024)
@@ -815,6 +1141,106 @@ hipError_t fused_elem_launch(const FusedInst *inst, const FusedBlock *blocks, in
 {
     hipLaunchKernelGGL(k_felem, dim3(nblocks), dim3(FUSED_ELEM_BLOCK), 0, s, inst, blocks, parity, step_rel,
                        nsteps);
+    return hipGetLastError();
+}
+
+// ---- 3D
+
+template <int NC, int MAXR>
+static hipError_t f3rows_n(const FusedInst *inst, long nrows, hipStream_t s, const float2 *sets)
+{
+    constexpr int T = FRowCfg<NC, 1, MAXR>::T;
+    constexpr int LPB = T >= 256 ? 1 : 256 / T;  // 256 threads a workgroup
+    static_assert(FRowCfg<NC, LPB, MAXR>::valid(), "3D row tile");
+    hipLaunchKernelGGL((k_f3rows<NC, LPB, MAXR>), dim3((unsigned)((nrows + LPB - 1) / LPB)),
+                       dim3(FRowCfg<NC, LPB, MAXR>::THREADS), 0, s, inst, sets);
+    return hipGetLastError();
+}
+
+hipError_t fused3d_rows_launch(int nc, int maxr, const FusedInst *inst, long nrows, hipStream_t s)
+{
+    const float2 *sets = twiddle_sets(s);
+    if (!sets) return hipErrorOutOfMemory;
+    switch (nc) {
+#define X(NC)                                                                 \
+    case NC:                                                                  \
+        if (maxr > 3) {                                                       \
+            if constexpr (FUSED_WIDE_NC(NC))                                  \
+                return f3rows_n<NC, FUSED_MAX_REAL>(inst, nrows, s, sets);    \
+            return hipErrorInvalidValue;                                      \
+        }                                                                     \
+        return f3rows_n<NC, 3>(inst, nrows, s, sets);
+        X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024)
+#undef X
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+#define F3Z_CASES(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024)
+#define F3Z_TWD(X) X(4) X(8) X(16) X(32) X(64)
+
+template <int N>
+static std::vector<int> z_shapes()
+{
+    std::vector<int> v;
+#define X(TWD) \
+    if (F3zCfg<N, TWD>::valid()) v.push_back(TWD);
+    F3Z_TWD(X)
+#undef X
+    return v;
+}
+
+std::vector<int> fused3d_z_shapes(int nz)
+{
+    switch (nz) {
+#define X(N) \
+    case N:  \
+        return z_shapes<N>();
+        F3Z_CASES(X)
+#undef X
+    default:
+        return {};
+    }
+}
+
+template <int N>
+static hipError_t f3z_n(int twd, const FusedInst *inst, int nc1, int sy, int parity, uint32_t step_rel, int mode,
+                        hipStream_t s, const float2 *sets, unsigned long long *ts)
+{
+#define X(TWD)                                                                                           \
+    if constexpr (F3zCfg<N, TWD>::valid()) {                                                            \
+        if (twd == TWD) {                                                                                \
+            const long total = (long)((nc1 + TWD - 1) / TWD) * sy;                                       \
+            hipLaunchKernelGGL((k_f3z<N, TWD>), dim3((unsigned)xcd_blocks(total)),                      \
+                               dim3(F3zCfg<N, TWD>::THREADS), 0, s, inst, parity, step_rel, mode, sets, ts); \
+            return hipGetLastError();                                                                    \
+        }                                                                                                \
+    }
+    F3Z_TWD(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t fused3d_z_launch(int nz, int twd, const FusedInst *inst, int nc1, int sy, int parity, uint32_t step_rel,
+                            int mode, hipStream_t s, unsigned long long *ts)
+{
+    const float2 *sets = twiddle_sets(s);
+    if (!sets) return hipErrorOutOfMemory;
+    switch (nz) {
+#define X(N) \
+    case N:  \
+        return f3z_n<N>(twd, inst, nc1, sy, parity, step_rel, mode, s, sets, ts);
+        F3Z_CASES(X)
+#undef X
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+hipError_t fused3d_derived_launch(const FusedInst *inst, long nh, int parity, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_f3der, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, s, inst, parity);
     return hipGetLastError();
 }
 

@@ -76,6 +76,7 @@ struct FusedInst {
     float2 *cnat[FUSED_MAX_CONS];             // their system buffers (natural layout)
     float2 *Z[FUSED_MAX_REAL];                // IFFT_y(mask_f * field_f), columns kx <= kmax[f]
     float2 *X[FUSED_MAX_GROUPS];              // R2C_x(sum of the products of group g)
+    float2 *W[FUSED_MAX_REAL];                // 3D: IFFT_z(mask_f * L_f * phi), natural layout
     float2 *eta;                              // R2C_x(xi) rows (NOISE_REAL)
     uint64_t step_base;                       // step counter of the first step of a launch sequence
     // row pass
@@ -132,6 +133,20 @@ hipError_t fused_cols_launch(int ny, int twd, const FusedInst *inst, const Fused
                              int parity, uint32_t step_rel, int mode, hipStream_t s, unsigned long long *ts = nullptr);
 // the slots of n timed launches to {~0, 0} (a kernel, so that graphs can hold it)
 hipError_t fused_ts_reset_launch(unsigned long long *ts, int n, hipStream_t s);
+// ---- 3D systems (one instance, natural half-spectrum layout; insts[0]).
+// A step is four passes: rows (fused3d_rows_launch), FFT_y of X_g
+// (lines_y), the z pass (fused3d_z_launch) and IFFT_y of W_f into Z_f
+// (lines_y).
+// z-line tile widths (kx lines per workgroup) for sz
+std::vector<int> fused3d_z_shapes(int nz);
+hipError_t fused3d_rows_launch(int nc, int maxr, const FusedInst *inst, long nrows, hipStream_t s);
+// mode 0: prologue (W_f of the current state), 1: step, 2: step without W_f
+// (the run's last); ts as for fused_cols_launch
+hipError_t fused3d_z_launch(int nz, int twd, const FusedInst *inst, int nc1, int sy, int parity, uint32_t step_rel,
+                            int mode, hipStream_t s, unsigned long long *ts = nullptr);
+// derived fields from phi's spectrum [parity], natural layout
+hipError_t fused3d_derived_launch(const FusedInst *inst, long nh, int parity, hipStream_t s);
+
 // natural spectra -> column-major buffer 0 (phi and the stored constraints;
 // blocks: one tile each)
 hipError_t fused_transpose_in_launch(const FusedInst *inst, const FusedBlock *blocks, int nblocks, hipStream_t s);

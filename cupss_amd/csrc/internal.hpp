@@ -96,6 +96,10 @@ hipError_t r2c(const Geom &g, const R2CSource &src, float2 *out, hipStream_t s);
 hipError_t c2r(const Geom &g, const float2 *in, int mask_p, float scale, float *out, float2 *scratch,
                hipStream_t s);
 hipError_t update(const UpdateArgs &a, hipStream_t s);
+// unnormalised C2C transforms (dir -1 forward, +1 inverse) of the lines along
+// y of a natural half spectrum with kx < ncols: line (kx, kz) holds
+// in[(kz*sy + ky)*nc1 + kx], ky < sy; out may be in
+hipError_t lines_y(const Geom &g, int dir, const float2 *in, float2 *out, long ncols, hipStream_t s);
 // eta_hat with the law of R2C(xi), xi ~ N(0,1) per cell, from Philox
 hipError_t spectral_noise_fill(const Geom &g, float2 *eta, uint64_t seed, uint64_t step, uint32_t stream,
                                hipStream_t s);

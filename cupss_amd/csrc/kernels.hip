@@ -423,6 +423,13 @@ hipError_t c2r(const Geom &g, const float2 *in, int mask_p, float scale, float *
     return rows_c2r(g, cur, out, scale, masked ? 0 : mask_p, s, tw);
 }
 
+hipError_t lines_y(const Geom &g, int dir, const float2 *in, float2 *out, long ncols, hipStream_t s)
+{
+    const float2 *tw = twiddles(s);
+    if (!tw) return hipErrorOutOfMemory;
+    return cols(g, g.sy, dir, false, 0, in, out, ncols, g.nc1, (long)g.nc1 * g.sy, g.sz, s, tw);
+}
+
 // ------------------------------------------------------------ update
 
 // Semi-implicit Euler update of one field on the half spectrum (one

@@ -119,6 +119,7 @@ extern "C" int cupss_system_destroy(cupss_system s)
     for (auto *p : s->fz) hipFree(p);
     for (auto *p : s->fx) hipFree(p);
     for (auto *p : s->ft) hipFree(p);
+    for (auto *p : s->fw) hipFree(p);
     for (auto &c : s->fc) {
         hipFree(c[0]);
         hipFree(c[1]);
@@ -682,7 +683,9 @@ extern "C" int cupss_system_advance(cupss_system s, int nsteps)
     if (int r = check_sys(s)) return r;
     if (!s->prepared) return fail(CUPSS_ERR_STATE, "cupss_system_prepare must be called before advancing");
     if (nsteps <= 0) return CUPSS_OK;
-    if (s->path_mode == 0 && s->plan.ok && !hooks_present(s)) {
+    // the 3D fused step draws noise in spectral space only
+    const bool real_noise_3d = s->g.sz > 1 && s->plan.ok && s->f[s->plan.phi].noisy && s->noise_mode == 0;
+    if (s->path_mode == 0 && s->plan.ok && !hooks_present(s) && !real_noise_3d) {
         // a stale dealiased copy is consumed by one general step first
         if (s->f[s->plan.phi].dstate == 3) {
             if (int r = general_steps(s, 1)) return r;
@@ -728,7 +731,9 @@ extern "C" int cupss_system_device_bytes(cupss_system s, size_t *bytes)
     }
     for (auto &T : s->t)
         if (T.xhat) b += spec;
-    if (s->plan.ok)  // the fused path's buffers, allocated at its first run
+    if (s->plan.ok && s->g.sz > 1)  // the 3D fused path's buffers (Z_f, W_f, X_g)
+        b += spec * (2 * s->plan.real.size() + s->plan.groups.size());
+    else if (s->plan.ok)  // the fused path's buffers, allocated at its first run
         b += spec * (2 + 2 * s->plan.cons.size() + s->plan.real.size() + s->plan.groups.size());
     *bytes = b;
     return CUPSS_OK;
@@ -754,7 +759,7 @@ extern "C" int cupss_system_set_noise_mode(cupss_system s, int mode)
 extern "C" int cupss_system_fused_eligible(cupss_system s)
 {
     if (check_sys(s)) return 0;
-    return s->plan.ok && !hooks_present(s) ? 1 : 0;
+    return s->plan.ok && s->g.sz == 1 && !hooks_present(s) ? 1 : 0;
 }
 
 // ------------------------------------------------------------ fused path
@@ -777,12 +782,13 @@ static bool same_monomial(const cupss_pres &a, const cupss_pres &b)
     return a.q2n == b.q2n && a.iqx == b.iqx && a.iqy == b.iqy && a.iqz == b.iqz && a.invq == b.invq;
 }
 
-// Does the fused 2D step apply, and how?  One dynamic field phi; every other
+// Does the fused step apply, and how?  One dynamic field phi; every other
 // field is either derived (psi = M_psi phi: one single-field term of phi,
 // no implicit operator, no noise) or stored (a constraint without noise
 // whose product terms read only derived or stored fields -- a product with
 // phi as a factor would need phi's dealiased copy one step early).  phi's
-// own terms may read any field.
+// own terms may read any field.  3D systems (their own four-pass step, run
+// alone) take no stored fields.
 static void plan_fused(cupss_system s)
 {
     auto &P = s->plan;
@@ -836,6 +842,7 @@ static void plan_fused(cupss_system s)
     for (int i = 0; i < nf; i++)
         if (stored[i]) P.cons.push_back(i);
     if ((int)P.cons.size() > FUSED_MAX_CONS) return;
+    if (s->g.sz > 1 && !P.cons.empty()) return;  // the 3D step has no stored fields
     // groups: phi's first, then each stored field's
     std::vector<int> order = {phi};
     order.insert(order.end(), P.cons.begin(), P.cons.end());

@@ -88,6 +88,7 @@ struct cupss_system_t {
     float2 *fx[cupss::FUSED_MAX_GROUPS] = {};
     float2 *ft[2] = {nullptr, nullptr};  // phi's spectrum, column-major, during fused runs
     float2 *fc[cupss::FUSED_MAX_CONS][2] = {};  // stored constraint fields, column-major
+    float2 *fw[cupss::FUSED_MAX_REAL] = {};     // 3D: IFFT_z of the masked real fields (zero off the kept lines)
     // After a fused run, fc[.][cont_buf] holds the stored fields' values of
     // the next step's phase A/B; a run that starts at step cont_step with
     // generation cont_gen continues from them.  Anything that changes the
