@@ -854,12 +854,51 @@ static int run(FusedBatch &B, int nsteps)
     return settle(B, nsteps);
 }
 
+// z-line tile width of a 3D system: every candidate timed on the system's own
+// buffers.  A z pass in mode 2 reads phi [par] and X_g and writes only the
+// other spectrum buffer, which the next step overwrites: the state is kept.
+static int tune3d(FusedBatch &B, cupss_system s, int par, uint32_t rel)
+{
+    const Geom &g = s->g;
+    const std::vector<int> v = fused3d_z_shapes(g.sz);
+    if (v.empty()) return fail(CUPSS_ERR_UNSUPPORTED, "no z-line tile for this size");
+    hipStream_t st = s->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    float best = 0;
+    int pick = v[0];
+    int rc = CUPSS_OK;
+    for (int w : v) {
+        float ms = 0;
+        hipError_t e = fused3d_z_launch(g.sz, w, B.d, g.nc1, g.sy, par, rel, 2, st);  // warm-up
+        if (e == hipSuccess) e = hipEventRecord(e0, st);
+        for (int r = 0; r < 3 && e == hipSuccess; r++) e = fused3d_z_launch(g.sz, w, B.d, g.nc1, g.sy, par, rel, 2, st);
+        if (e == hipSuccess) e = hipEventRecord(e1, st);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+            rc = fail(CUPSS_ERR_HIP, std::string("z-pass tuning: ") + hipGetErrorString(e));
+            break;
+        }
+        if (w == v[0] || ms < best) {
+            best = ms;
+            pick = w;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc == CUPSS_OK) B.twd3 = pick;
+    return rc;
+}
+
 // The fused step of a 3D system (fused.hip: k_f3rows, k_f3z; the y lines
 // by lines_y), natural layout, no transposes.  A run is
-//   prologue:  z pass in mode 0 and IFFT_y: W_f, Z_f of the current state;
+//   prologue:  z pass in mode 0 and IFFT_y: W_f, Z_f of the current state
+//              (skipped when the previous run left them and nothing changed
+//              the system since: per-step callers pay no prologue);
 //   steps:     rows, FFT_y of every group, z pass (update; W_f of the new
-//              state except in the run's last step), IFFT_y of every real
-//              field;
+//              state), IFFT_y of every real field;
 //   end:       the derived fields of the last step's starting phi.
 // The descriptor keeps both spectrum buffers of phi in a fixed order (the
 // launches pass the parity) and is uploaded again only when it changes.
@@ -912,19 +951,17 @@ static int run3d(FusedBatch &B, cupss_system s, int nsteps)
         B.h.assign(1, a);
         B.d3 = true;
     }
-    if (!B.twd3) {
-        const std::vector<int> v = fused3d_z_shapes(g.sz);
-        if (v.empty()) return fail(CUPSS_ERR_UNSUPPORTED, "no z-line tile for this size");
-        B.twd3 = v[0];
-        for (int w : v)
-            if (w <= 8) B.twd3 = w;
-    }
     if (!twiddle_sets(s->stream) || !twiddles(s->stream)) return fail(CUPSS_ERR_NOMEM, "twiddle tables");
     const uint32_t rel0 = (uint32_t)(s->step - B.h[0].step_base);
     const int par0 = F.cur, maxr = a.nreal > 3 ? FUSED_MAX_REAL : 3;
     const long nrows = (long)g.sy * g.sz;
     hipStream_t st = s->stream;
-    if (a.nreal > 0) {
+    if (!B.twd3)
+        if (int r = tune3d(B, s, par0, rel0)) return r;
+    // Z_f of the current state: left by the previous run when nothing changed
+    // the system since (each run ends with them), else the prologue
+    const bool cont = s->cont_step == s->step && s->cont_gen == s->gen;
+    if (a.nreal > 0 && !cont) {
         HIP_TRY(fused3d_z_launch(g.sz, B.twd3, B.d, g.nc1, g.sy, par0, rel0, 0, st));
         for (int f = 0; f < a.nreal; f++) HIP_TRY(lines_y(g, +1, s->fw[f], s->fz[f], a.kmax[f] + 1, st));
     }
@@ -934,7 +971,6 @@ static int run3d(FusedBatch &B, cupss_system s, int nsteps)
         HIP_TRY(fused_ts_reset_launch(B.pts, FusedBatch::PROF_SLOTS, st));
     }
     for (int i = 0; i < nsteps; i++) {
-        const bool last = i == nsteps - 1;
         if (a.ngroups > 0) {
             HIP_TRY(fused3d_rows_launch(g.nc, maxr, B.d, nrows, st));
             for (int gi = 0; gi < a.ngroups; gi++) HIP_TRY(lines_y(g, -1, s->fx[gi], s->fx[gi], g.nc1, st));
@@ -942,15 +978,15 @@ static int run3d(FusedBatch &B, cupss_system s, int nsteps)
         unsigned long long *ts = nullptr;
         if (B.profile && B.pcount < FusedBatch::PROF_SLOTS)
             ts = B.pts + (size_t)(B.pcount++) * FUSED_TS_SLOTS * FUSED_TS_STRIDE;
-        HIP_TRY(fused3d_z_launch(g.sz, B.twd3, B.d, g.nc1, g.sy, par0 ^ (i & 1), rel0 + (uint32_t)i, last ? 2 : 1, st,
-                                 ts));
-        if (!last)
-            for (int f = 0; f < a.nreal; f++) HIP_TRY(lines_y(g, +1, s->fw[f], s->fz[f], a.kmax[f] + 1, st));
+        HIP_TRY(fused3d_z_launch(g.sz, B.twd3, B.d, g.nc1, g.sy, par0 ^ (i & 1), rel0 + (uint32_t)i, 1, st, ts));
+        for (int f = 0; f < a.nreal; f++) HIP_TRY(lines_y(g, +1, s->fw[f], s->fz[f], a.kmax[f] + 1, st));
     }
     if (a.nder > 0) HIP_TRY(fused3d_derived_launch(B.d, g.nh, par0 ^ ((nsteps - 1) & 1), st));
     F.cur = par0 ^ (nsteps & 1);
     s->step += (uint64_t)nsteps;
     s->fused_steps += nsteps;
+    s->cont_step = s->step;
+    s->cont_gen = s->gen;
     if (F.order > 0) F.dstate = 2;
     for (int dfi : P.derived)
         if (s->f[dfi].order > 0) s->f[dfi].dstate = 2;

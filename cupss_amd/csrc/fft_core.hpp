@@ -212,13 +212,19 @@ struct Stockham {
                     for (int r = 1; r < R; r++) a[r] = cmul(a[r], tw_lookup<DIR, TWS>(tw, r * k, NS * R));
                 }
                 Dft<R, DIR>::run(a);
-                const int base = (j - k) * R + k;
+                // lpad(base + r*NS) = lpad(base) + lpad(r*NS): base = (j-k)*R + k
+                // with (j-k)*R a multiple of NS*R, so the R outputs either stay
+                // inside one 32-entry block (NS*R <= 32) or step across blocks
+                // exactly as r*NS does.  One address register, constant offsets.
+                const int wb = lpad((j - k) * R + k);
 #pragma unroll
-                for (int r = 0; r < R; r++) lds[lpad(base + r * NS)] = a[r];
+                for (int r = 0; r < R; r++) lds[wb + lpad(r * NS)] = a[r];
             }
             __syncthreads();
+            // lpad(t + e*T) = lpad(t) + lpad(e*T) for t < T, T a power of two
+            const int rb = lpad(t);
 #pragma unroll
-            for (int e = 0; e < E; e++) v[e] = lds[lpad(t + e * T)];
+            for (int e = 0; e < E; e++) v[e] = lds[rb + lpad(e * T)];
             if constexpr (NS * R < N) __syncthreads();
             Stockham<N, E, DIR, NS * R, TWS>::run(v, lds, t, tw);
         }

@@ -814,8 +814,10 @@ struct F3zCfg {
 
 // Lines along z of TWD adjacent kx at one ky (thread = t*TWD + c: a row of
 // TWD values is one contiguous access).  Workgroups are placed XCD-aware.
+// Register budget for 4 waves per SIMD (128 VGPRs): the pass waits on
+// strided HBM accesses, which more resident waves hide.
 template <int N, int TWD>
-__global__ __launch_bounds__((F3zCfg<N, TWD>::THREADS)) void k_f3z(const FusedInst *__restrict__ insts, int parity,
+__global__ __launch_bounds__((F3zCfg<N, TWD>::THREADS)) __attribute__((amdgpu_waves_per_eu(4))) void k_f3z(const FusedInst *__restrict__ insts, int parity,
                                                                  uint32_t step_rel, int mode,
                                                                  const float2 *__restrict__ sets,
                                                                  unsigned long long *__restrict__ ts)

@@ -1,6 +1,7 @@
 // fused.hpp -- the fused two-kernel step for 2D systems with one dynamic
 // field phi (fused.hip), batched over any number of independent instances
-// (batch.hip).
+// (batch.hip), and the four-pass fused step of such 3D systems (one at a
+// time).
 //
 // The other fields of such a system are
 //   derived:  psi = M_psi phi (one single-field term of phi, no implicit
@@ -141,7 +142,7 @@ hipError_t fused_ts_reset_launch(unsigned long long *ts, int n, hipStream_t s);
 std::vector<int> fused3d_z_shapes(int nz);
 hipError_t fused3d_rows_launch(int nc, int maxr, const FusedInst *inst, long nrows, hipStream_t s);
 // mode 0: prologue (W_f of the current state), 1: step, 2: step without W_f
-// (the run's last); ts as for fused_cols_launch
+// (timing); ts as for fused_cols_launch
 hipError_t fused3d_z_launch(int nz, int twd, const FusedInst *inst, int nc1, int sy, int parity, uint32_t step_rel,
                             int mode, hipStream_t s, unsigned long long *ts = nullptr);
 // derived fields from phi's spectrum [parity], natural layout

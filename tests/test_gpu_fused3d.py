@@ -7,7 +7,8 @@
   deterministic CH, KPZ with three derived gradient fields and spectral
   noise (both paths draw the same Philox stream per bin), and the derived
   fields written back after a run.
-* Split runs equal one run bit for bit (prologue equivalence).
+* Split runs and one step per call equal one run bit for bit (prologue
+  equivalence, and runs that continue from the previous run's real fields).
 * Mass conservation at the full 256^3 size (every CH term carries q^2).
 * A noisy 3D system in real-space noise mode runs the general program.
 """
@@ -97,6 +98,14 @@ def test_split_runs_equal_one_run_3d():
         b.step(1)
         b.step(12)
         assert np.array_equal(a.get_real(ia[key]), b.get_real(ib[key]))
+        # one step per call (the C++ evolver's advanceTime), reading the
+        # state in between
+        c, ic = gpu(spec, noise_mode=1)
+        for i in range(23):
+            c.step(1)
+            if i in (4, 17):
+                c.get_real(ic[key])
+        assert np.array_equal(a.get_real(ia[key]), c.get_real(ic[key]))
 
 
 def test_real_space_noise_3d_runs_general_program():
