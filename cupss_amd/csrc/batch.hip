@@ -414,7 +414,13 @@ static int class_tables(const FusedBatch &B, const FusedBatch::Class &c, int lpb
         for (int i : part) {
             const Geom &g = B.h[i].g;
             for (int r0 = 0; r0 < g.sy; r0 += lpb) rows.push_back(FusedBlock{i, r0});
-            for (int c0 = 0; c0 < g.nc1; c0 += twd) cols.push_back(FusedBlock{i, c0});
+            // column tiles interleaved with stride 8: each XCD takes a
+            // contiguous range of the table, and the columns the dealias
+            // masks keep (which also transform Z_f) sit at low kx; a stride-8
+            // order gives every XCD its share of them
+            const int ntile = (g.nc1 + twd - 1) / twd;
+            for (int j = 0; j < 8; j++)
+                for (int tc = j; tc < ntile; tc += 8) cols.push_back(FusedBlock{i, tc * twd});
         }
         out.push_back(Tables());
         if (int r = upload_blocks(rows, &out.back().rb, &out.back().nrb)) return r;

@@ -833,7 +833,11 @@ __global__ __launch_bounds__((F3zCfg<N, TWD>::THREADS)) __attribute__((amdgpu_wa
     if (item >= (long)ntx * g.sy) return;  // whole workgroup idle
     unsigned long long *slot = ts ? ts + (size_t)(blockIdx.x % FUSED_TS_SLOTS) * FUSED_TS_STRIDE : nullptr;
     if (slot && threadIdx.x == 0) atomicMin(&slot[0], (unsigned long long)wall_clock64());
-    const int kx0 = (int)(item % ntx) * TWD, ky = (int)(item / ntx);
+    // Each XCD takes a contiguous range of items; the lines the dealias masks
+    // keep (which also transform W_f) have small |ky|, so ky is dealt with
+    // stride 8 over the ranges to give every XCD its share of them.
+    const int kr = (int)(item / ntx), yq = g.sy >> 3;
+    const int kx0 = (int)(item % ntx) * TWD, ky = (kr % yq) * 8 + kr / yq;
     const int c = threadIdx.x % TWD, t = threadIdx.x / TWD;
     const int kx = kx0 + c;
     const bool active = kx < g.nc1;
