@@ -45,7 +45,8 @@ def _compile(src):
 # C++ programs linked against libcupss.so: the test drivers of tests/cpp and
 # the example solvers of examples/ (binaries land in a bin/ folder beside them)
 TOOLS = [os.path.join(ROOT, "tests", "cpp", "reference_cases.cpp"),
-         os.path.join(ROOT, "tests", "cpp", "system_driver.cpp")]
+         os.path.join(ROOT, "tests", "cpp", "system_driver.cpp"),
+         os.path.join(ROOT, "tests", "cpp", "hooks_driver.hip")]
 
 
 def _tool_sources():

@@ -1081,7 +1081,8 @@ extern "C" int cupss_batch_add(cupss_batch b, cupss_system s)
     if (!s->prepared) return fail(CUPSS_ERR_STATE, "prepare the system before adding it to a batch");
     if (!s->plan.ok || s->g.sz > 1) return fail(CUPSS_ERR_UNSUPPORTED, "system does not qualify for the fused batched step");
     for (auto &F : s->f)
-        if (F.real_hook || F.spec_hook) return fail(CUPSS_ERR_UNSUPPORTED, "systems with callbacks cannot be batched");
+        if (F.real_hook || F.spec_hook || F.real_dev || F.spec_dev)
+            return fail(CUPSS_ERR_UNSUPPORTED, "systems with callbacks cannot be batched");
     if (s->f[s->plan.phi].dstate == 3) {
         // consume a stale dealiased copy with one general step, as a lone system would
         if (int r = general_steps(s, 1)) return r;

@@ -72,6 +72,22 @@ void spec_hook(void *user, int, void *full2)
     abi_check(cupss_memcpy_h2d(full2, f->comp_array, n * sizeof(float2)), "callback copy");
 }
 
+// device-side callbacks: the library's buffers and stream, passed through
+void real_hook_dev(void *user, int, float *real, int sx, int sy, int sz, int which, void *stream)
+{
+    field *f = static_cast<field *>(user);
+    if (f->hasCBDevice && f->callbackDevice)
+        f->callbackDevice(f->system_p, real, sx, sy, sz, which, static_cast<hipStream_t>(stream));
+}
+
+void spec_hook_dev(void *user, int, void *half2, int sx, int sy, int sz, void *stream)
+{
+    field *f = static_cast<field *>(user);
+    if (f->hasCBFourierDevice && f->callbackFourierDevice)
+        f->callbackFourierDevice(f->system_p, static_cast<float2 *>(half2), sx, sy, sz,
+                                 static_cast<hipStream_t>(stream));
+}
+
 }  // namespace
 
 void evolver::common_constructor()
@@ -162,6 +178,12 @@ void evolver::syncHooks()
         abi_check(cupss_field_set_hooks(device_system, f->device_id, r ? real_hook : nullptr,
                                         c ? spec_hook : nullptr, f),
                   "cupss_field_set_hooks");
+        // device-side callbacks act on device buffers: GPU systems only
+        const bool rd = with_cuda && f->hasCBDevice && f->callbackDevice;
+        const bool cd = with_cuda && f->hasCBFourierDevice && f->callbackFourierDevice;
+        abi_check(cupss_field_set_device_hooks(device_system, f->device_id, rd ? real_hook_dev : nullptr,
+                                               cd ? spec_hook_dev : nullptr, f),
+                  "cupss_field_set_device_hooks");
     }
 }
 

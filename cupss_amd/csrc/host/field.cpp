@@ -62,6 +62,10 @@ void field::common_constructor()
     callback = NULL;
     hasCBFourier = false;
     callbackFourier = NULL;
+    hasCBDevice = false;
+    callbackDevice = NULL;
+    hasCBFourierDevice = false;
+    callbackFourierDevice = NULL;
     isNoisy = false;
     noiseType = GaussianWhite;
     integrator = EULER;

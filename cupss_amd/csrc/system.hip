@@ -226,9 +226,26 @@ extern "C" int cupss_field_set_hooks(cupss_system s, int field, cupss_real_hook 
                                      cupss_spec_hook spec_hook, void *user)
 {
     if (int r = check_field(s, field)) return r;
-    s->f[field].real_hook = real_hook;
-    s->f[field].spec_hook = spec_hook;
-    s->f[field].hook_user = user;
+    FieldRec &F = s->f[field];
+    // the C++ evolver re-registers its hooks before every step: only a
+    // change alters the step program (and invalidates plans and graphs)
+    if (F.real_hook == real_hook && F.spec_hook == spec_hook && F.hook_user == user) return CUPSS_OK;
+    F.real_hook = real_hook;
+    F.spec_hook = spec_hook;
+    F.hook_user = user;
+    s->gen++;
+    return CUPSS_OK;
+}
+
+extern "C" int cupss_field_set_device_hooks(cupss_system s, int field, cupss_real_hook_dev real_hook,
+                                            cupss_spec_hook_dev spec_hook, void *user)
+{
+    if (int r = check_field(s, field)) return r;
+    FieldRec &F = s->f[field];
+    if (F.real_dev == real_hook && F.spec_dev == spec_hook && F.dev_user == user) return CUPSS_OK;
+    F.real_dev = real_hook;
+    F.spec_dev = spec_hook;
+    F.dev_user = user;
     s->gen++;
     return CUPSS_OK;
 }
@@ -483,6 +500,24 @@ static int post_update(cupss_system s, int fi)
     const Geom &g = s->g;
     const float norm = 1.0f / (float)g.n;
     float2 *spec = F.spec[F.cur];
+    // device-side hooks: the library's own buffers, stream-ordered
+    if (F.spec_dev) F.spec_dev(F.dev_user, fi, spec, g.sx, g.sy, g.sz, (void *)s->stream);
+    if (F.real_dev) {
+        HIP_TRY(c2r(g, spec, 0, norm, s->rtmp, s->scratch, s->stream));
+        F.real_dev(F.dev_user, fi, s->rtmp, g.sx, g.sy, g.sz, 0, (void *)s->stream);
+        if (F.order > 0) {
+            HIP_TRY(c2r(g, spec, F.order, norm, F.dreal, s->scratch, s->stream));
+            F.real_dev(F.dev_user, fi, F.dreal, g.sx, g.sy, g.sz, 1, (void *)s->stream);
+        }
+        HIP_TRY(hipGetLastError());
+        R2CSource src{};
+        src.kind = LOAD_REAL;
+        src.f[0] = s->rtmp;
+        src.nf = 1;
+        HIP_TRY(r2c(g, src, spec, s->stream));
+        if (F.order > 0) F.dstate = 1;
+        if (!F.spec_hook && !F.real_hook) return CUPSS_OK;
+    }
     if (F.spec_hook) {
         hipLaunchKernelGGL(k_half_to_full, dim3(nblk(g.n)), dim3(256), 0, s->stream, spec, s->full, g);
         HIP_TRY(hipStreamSynchronize(s->stream));
@@ -607,7 +642,7 @@ int cupss::general_steps(cupss_system s, int nsteps)
 static bool hooks_present(cupss_system s)
 {
     for (auto &F : s->f)
-        if (F.real_hook || F.spec_hook) return true;
+        if (F.real_hook || F.spec_hook || F.real_dev || F.spec_dev) return true;
     return false;
 }
 

@@ -40,6 +40,9 @@ struct FieldRec {
     cupss_real_hook real_hook = nullptr;
     cupss_spec_hook spec_hook = nullptr;
     void *hook_user = nullptr;
+    cupss_real_hook_dev real_dev = nullptr;   // device-side hooks (fast path)
+    cupss_spec_hook_dev spec_dev = nullptr;
+    void *dev_user = nullptr;
 };
 
 struct TermRec {

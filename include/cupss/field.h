@@ -97,6 +97,16 @@ class field {
     void (*callback) (evolver *, float2 *, int, int, int);
     bool hasCBFourier;
     void (*callbackFourier) (evolver *, float2 *, int, int, int);
+    // Device-side callbacks, the fast path of the two above (no counterpart
+    // in the reference): they get the library's own device buffers -- the
+    // real values as float[sz][sy][sx] (which = 0: the field, 1: its
+    // dealiased copy), or the half spectrum float2[sz][sy][sx/2+1] -- and
+    // the evolver's HIP stream, on which they must launch their work.  No
+    // float2 copies and no device synchronisation per call.
+    bool hasCBDevice;
+    void (*callbackDevice) (evolver *, float *, int, int, int, int, hipStream_t);
+    bool hasCBFourierDevice;
+    void (*callbackFourierDevice) (evolver *, float2 *, int, int, int, hipStream_t);
 
     dim3 threads_per_block;
     dim3 blocks;

@@ -66,6 +66,23 @@ typedef void (*cupss_real_hook)(void *user, int field, void *real2, int which);
  * full C2C spectrum; the hook may change it (the change is then projected
  * onto real fields, as the reference's IFFT/normalize/FFT does). */
 typedef void (*cupss_spec_hook)(void *user, int field, void *full2);
+/* Device-side hooks: the fast path of the two hooks above (no counterpart
+ * in the reference, whose callbacks always receive float2 arrays; for
+ * boundary-condition extensions such as its examples 07 and 08).  They are
+ * called with the library's own device buffers and its HIP stream and must
+ * enqueue their work on that stream: no float2 copies, no host
+ * synchronisation.
+ *   real: `real` holds the field's real values, float[sz][sy][sx] (x
+ *         fastest); which = 0 for the field (then transformed back to the
+ *         spectrum), 1 for its dealiased copy; changed in place.
+ *   spectral: `half2` is the field's half spectrum, float2[sz][sy][sx/2+1]
+ *         (the R2C layout: bin (kx, ky, kz) at (kz*sy + ky)*(sx/2+1) + kx),
+ *         changed in place.
+ * They run after the update of the field, in the reference's order
+ * (spectral, then real), before the host hooks above when both are set. */
+typedef void (*cupss_real_hook_dev)(void *user, int field, float *real, int sx, int sy, int sz, int which,
+                                    void *hip_stream);
+typedef void (*cupss_spec_hook_dev)(void *user, int field, void *half2, int sx, int sy, int sz, void *hip_stream);
 
 const char *cupss_last_error(void);
 int cupss_abi_version(void);
@@ -97,6 +114,10 @@ int cupss_field_set_noise(cupss_system s, int field, cupss_pres amplitude, uint6
  * null hooks switch them off. */
 int cupss_field_set_hooks(cupss_system s, int field, cupss_real_hook real_hook, cupss_spec_hook spec_hook,
                           void *user);
+/* The device-side hooks of a field (see cupss_real_hook_dev); null hooks
+ * switch them off.  No counterpart in the reference's API. */
+int cupss_field_set_device_hooks(cupss_system s, int field, cupss_real_hook_dev real_hook,
+                                 cupss_spec_hook_dev spec_hook, void *user);
 
 /* field::copyHostToDevice + toComp (src/field.cpp:332-335, 261-273):
  * `host` holds N floats (the .x of real_array). */

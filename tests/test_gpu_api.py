@@ -6,6 +6,9 @@ initializeFromFile / advanceTime / writeOut) on the HIP path.
 * tests/cpp/system_driver: systems read from reference-format system files,
   stepped through advanceTime and written with the reference CSV output,
   against the oracle's FP32 run of the same parsed system.
+* tests/cpp/hooks_driver: field callbacks, the reference's float2 kind and
+  the device-side kind, give the same fields (mirror boundary conditions of
+  the reference's example 07; a Fourier-space low-pass filter).
 """
 import os
 import subprocess
@@ -27,6 +30,11 @@ def run(args, cwd, timeout=240):
     r = subprocess.run(args, cwd=cwd, capture_output=True, text=True, timeout=timeout)
     print(r.stdout[-3000:], r.stderr[-2000:], flush=True)
     return r
+
+
+def test_device_callbacks_match_float2_callbacks(tmp_path):
+    r = run([os.path.join(BIN, "hooks_driver")], str(tmp_path))
+    assert r.returncode == 0, r.stdout
 
 
 def test_reference_cases(tmp_path):
