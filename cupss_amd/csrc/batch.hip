@@ -74,6 +74,7 @@ struct FusedBatch {
     FusedBlock *tb = nullptr;   // transpose tiles of the transform instances
     int ntb = 0;
     hipEvent_t done = nullptr;  // recorded after each run; member streams wait on it
+    hipEvent_t staged = nullptr;  // recorded after each descriptor upload
 
     std::vector<FusedInst> h;
     FusedInst *d = nullptr;
@@ -136,6 +137,7 @@ struct FusedBatch {
 
     void release_tables()
     {
+        if (stream) hipStreamSynchronize(stream);  // launches in flight may still read the tables
         for (auto &c : classes) free_tables(c.chunks);
         classes.clear();
         hipFree(eb);
@@ -170,6 +172,8 @@ struct FusedBatch {
         dcap = 0;
         if (done) hipEventDestroy(done);
         done = nullptr;
+        if (staged) hipEventDestroy(staged);
+        staged = nullptr;
         if (own_stream && stream) hipStreamDestroy(stream);
         stream = nullptr;
     }
@@ -350,10 +354,12 @@ static void run_view(cupss_system s, bool elementwise, FusedInst &a)
         a.spec[1] = F.spec[F.cur ^ 1];
         a.nat = nullptr;
     } else {
-        a.spec[0] = s->ft[0];
-        a.spec[1] = s->ft[1];
+        const int b = s->resume_now ? s->cont_ft : 0;
+        a.spec[0] = s->ft[b];
+        a.spec[1] = s->ft[b ^ 1];
         a.nat = F.spec[F.cur];
     }
+    a.resume = !elementwise && s->resume_now ? 1 : 0;
     a.step_base = s->step;
     a.cont = s->cont_now ? 1 : 0;
     const int b0 = s->cont_now ? s->cont_buf : 0;
@@ -509,14 +515,18 @@ static bool stale_tables(const FusedBatch &B)
     return false;
 }
 
-// upload the per-run descriptors; the previous run must be finished with
-// the device table and the staging buffer before either is rewritten
+// upload the per-run descriptors.  The device table is rewritten in stream
+// order, after the previous run's kernels; the staging buffer may still feed
+// the previous upload, so only that copy (which ran at the start of the
+// previous run) is waited for -- the host does not wait for the steps.
 static int refresh(FusedBatch &B)
 {
-    HIP_TRY(hipStreamSynchronize(B.stream));
+    if (B.staged) HIP_TRY(hipEventSynchronize(B.staged));
     for (size_t i = 0; i < B.sys.size(); i++) run_view(B.sys[i], B.is_elem[i], B.h[i]);
     std::memcpy(B.pinned, B.h.data(), sizeof(FusedInst) * B.h.size());
     HIP_TRY(hipMemcpyAsync(B.d, B.pinned, sizeof(FusedInst) * B.h.size(), hipMemcpyHostToDevice, B.stream));
+    if (!B.staged) HIP_TRY(hipEventCreateWithFlags(&B.staged, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(B.staged, B.stream));
     return CUPSS_OK;
 }
 
@@ -785,6 +795,10 @@ static int settle(FusedBatch &B, int nsteps)
         if (!P.cons.empty()) {
             // the last pass left the next step's phase A/B in buffer nsteps&1
             s->cont_buf = (s->cont_now ? s->cont_buf : 0) ^ (nsteps & 1);
+        }
+        if (!B.is_elem[i]) {
+            // phi (column-major) and Z_f of the final state stay for a resume
+            s->cont_ft = (s->resume_now ? s->cont_ft : 0) ^ (nsteps & 1);
             s->cont_step = s->step;
             s->cont_gen = s->gen;
         }
@@ -809,7 +823,8 @@ static int run(FusedBatch &B, int nsteps)
     const bool will_tune = B.autotune && !B.tuned && !B.classes.empty();
     bool general = false;
     for (cupss_system s : B.sys) {
-        s->cont_now = !s->plan.cons.empty() && !will_tune && s->cont_step == s->step && s->cont_gen == s->gen;
+        s->resume_now = !will_tune && s->cont_step == s->step && s->cont_gen == s->gen;
+        s->cont_now = !s->plan.cons.empty() && s->resume_now;
         if (!s->plan.cons.empty() && !s->cont_now) {
             if (int r = general_constraint_phase(s)) return r;
             general = true;

@@ -330,6 +330,7 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
     __shared__ float2 lds[TWD * C::LL];
     const FusedBlock blk = blocks[blockIdx.x];
     if (blk.inst < 0) return;
+    if (mode == 0 && insts[blk.inst].resume) return;  // Z_f of this state left by the previous run
     // launch timing (ts != null): earliest workgroup start and latest end in
     // ticks of the constant-rate wall clock, spread over FUSED_TS_SLOTS
     // cache lines so that the workgroups' atomics do not contend
@@ -548,7 +549,7 @@ __global__ __launch_bounds__(256) void k_ftrans_in(const FusedInst *__restrict__
     const FusedInst &a = insts[blk.inst];
     int x0, y0;
     tile_origin(a.g, blk.line, x0, y0);
-    tile_to_cm(s, a.g, x0, y0, a.nat, a.spec[0]);
+    if (!a.resume) tile_to_cm(s, a.g, x0, y0, a.nat, a.spec[0]);
     if (!a.cont)
         for (int c = 0; c < a.ncons; c++) tile_to_cm(s, a.g, x0, y0, a.cnat[c], a.cs[c][0]);
 }

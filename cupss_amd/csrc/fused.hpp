@@ -93,6 +93,7 @@ struct FusedInst {
     UpdateArgs u;                             // phi: dt, noise, implicit D(q) (and its terms, elementwise kernel)
     int ncons;
     int cont;                                 // cs[.][0] already holds this run's first phase A/B
+    int resume;                               // spec[0] and Z_f already hold this run's start (no prologue)
     FusedTarget tgt[FUSED_MAX_TARGETS];
     TermRefDev gmul[FUSED_MAX_GROUPS];        // multiplier of each group (pool)
     PresDev pool[FUSED_MAX_POOL];

@@ -100,6 +100,11 @@ struct cupss_system_t {
     uint64_t cont_gen = 0;
     int cont_buf = 0;
     bool cont_now = false;                        // the current run continues
+    // 2D fused path: after a run, ft[cont_ft] holds phi column-major and Z_f
+    // the real fields of that state; a run that starts at cont_step with
+    // generation cont_gen resumes from them (no transpose in, no prologue)
+    int cont_ft = 0;
+    bool resume_now = false;
     long fused_steps = 0;     // steps taken by the fused path (diagnostics)
     uint64_t step = 0;
     int device = 0;

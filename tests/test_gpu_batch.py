@@ -70,15 +70,33 @@ def test_batch_equals_alone(noise_mode, graph):
 
 
 def test_split_runs_equal_one_run():
-    """Advancing 10 + 13 steps equals advancing 23 (prologue equivalence)."""
-    for sp in (spec_cahn_hilliard((64, 64, 1), seed=2), spec_kpz(64, nu=1.1, lam=1.3, D=0.01, dt=0.005, seed=5)):
+    """Advancing 10 + 13 steps, or one step per call (the C++ evolver's
+    advanceTime), equals advancing 23: runs that resume from the previous
+    run's column-major state and real fields, and runs after a state read
+    or a state write (which start over from the natural spectrum)."""
+    for sp in (spec_cahn_hilliard((64, 64, 1), seed=2), spec_kpz(64, nu=1.1, lam=1.3, D=0.01, dt=0.005, seed=5),
+               spec_model_h(64, seed=6)):
         key = "phi" if "phi" in dict(sp["fields"]) else "h"
         a, ia = make(sp)
         a.step(23)
+        ref = a.get_real(ia[key])
         b, ib = make(sp)
         b.step(10)
         b.step(13)
-        assert np.array_equal(a.get_real(ia[key]), b.get_real(ib[key]))
+        assert np.array_equal(ref, b.get_real(ib[key]))
+        c, ic = make(sp)
+        for i in range(23):
+            c.step(1)
+            if i == 6:
+                c.get_real(ic[key])
+            if i == 15:  # write the state back unchanged: the next run starts over
+                c.set_real(ic[key], c.get_real(ic[key]))
+        got = c.get_real(ic[key])
+        assert float(np.max(np.abs(got - ref))) <= 1e-5 * max(1.0, float(np.max(np.abs(ref))))
+        d, idd = make(sp)
+        for i in range(23):
+            d.step(1)
+        assert np.array_equal(ref, d.get_real(idd[key]))
 
 
 def ew_structure_factor(n, nu, D, dt, noise_mode, steps, burn, every=1):
