@@ -147,16 +147,31 @@ def bench_ensemble(args, C):
                              "graph_steps": args.graph}, kernel
 
 
-def roofline(value, kernel, per):
-    """The dominant kernel's roofline entry: the column launch that performs
-    the state update (one read and one write of each grid point's spectrum:
+# HBM bytes per launch of the dominant kernel of each workload (read + written),
+# measured with rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by the same
+# counters on a 256 MiB device copy (profiles/hbm_traffic_pmc.md has the
+# commands and the per-pass figures); keyed by workload and kernel template
+PMC_TRAFFIC = {
+    ("ensemble", "k_fcols<512,"): 1.49e8 + 1.70e8,   # 69 KPZ instances of 512^2 (seed 0)
+    ("ch2d", "k_fcols<1024,"): 8.89e6 + 6.32e6,
+    ("ch3d", "k_f3z<256,"): 1.56e8 + 1.00e8,
+}
+
+
+def roofline(value, kernel, per, workload=None):
+    """The dominant kernel's roofline entry: the launch that performs the
+    state update (one read and one write of each grid point's spectrum:
     8 B per point) of the class with the most points, its average duration
-    from HIP events recorded around its launches in the timed run."""
+    from the wall-clock stamps its workgroups record inside the timed run."""
     ms, pts, name = kernel
     achieved = BYTES_PER_POINT_STEP * pts / (ms / 1e3) / 1e9
     whole = BYTES_PER_POINT_STEP * value / 1e9
+    traffic = None
+    for (wl, prefix), b in PMC_TRAFFIC.items():
+        if wl == workload and name.replace(" ", "").startswith(prefix):
+            traffic = b
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": name,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": name,
             "kernel_avg_us": ms * 1e3, "kernel_points": pts,
             "whole_step_achieved": whole, "whole_step_frac": whole / HBM_PEAK_GBS, "per": per}
 
@@ -365,7 +380,8 @@ def main():
                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
                "data": "synthetic", "config": dict({"workload": wl["name"], "dt": 0.005,
                                                      "path": "batched fused step"}, **extra),
-               "roofline": roofline(value, kernel, "8 B per grid point of the timed launch (the state update)")}
+               "roofline": roofline(value, kernel, "8 B per grid point of the timed launch (the state update)",
+                                    "ensemble")}
         if not args.no_cpu:
             out["cpu_baseline"] = cpu_baseline_ensemble(ensemble_members())
         print(json.dumps(out), flush=True)
@@ -427,7 +443,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl["name"], "grid": list(dims), "dt": 0.01, "path": path,
                    "ms_per_step_one_step_per_call": per_call_ms},
-        "roofline": (roofline(value, kernel, "8 B per grid point of the timed launch (the state update)")
+        "roofline": (roofline(value, kernel, "8 B per grid point of the timed launch (the state update)",
+                              args.workload)
                      if kernel is not None else
                      {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": achieved / HBM_PEAK_GBS, "traffic": None,
