@@ -42,10 +42,12 @@ namespace cupss {
 // ------------------------------------------------------------ row kernel
 
 // MAXR: real fields a row holds at once (3, or FUSED_MAX_REAL for systems
-// with more; those use 4 elements per lane to stay within the registers)
+// with more; those use 4 elements per lane to stay within the registers).
+// Rows of 1024 real points also take 4 per lane (more waves for one large
+// system).
 template <int NC, int LPB, int MAXR = 3>
 struct FRowCfg {
-    static constexpr int E = MAXR > 3 ? (NC >= 4 ? 4 : NC) : (NC >= 1024 ? 16 : (NC >= 64 ? 8 : (NC >= 4 ? 4 : NC)));
+    static constexpr int E = MAXR > 3 ? (NC >= 4 ? 4 : NC) : (NC >= 1024 ? 16 : (NC >= 512 ? 4 : (NC >= 64 ? 8 : (NC >= 4 ? 4 : NC))));
     static constexpr int T = NC / E;
     static constexpr int THREADS = LPB * T;
     static constexpr int RS = lds_len(NC);  // tile row stride (>= NC + 1), odd for NC >= 32
@@ -310,7 +312,9 @@ __device__ __forceinline__ void z_out(const FusedInst &a, int f, const Geom &g, 
 
 template <int N, int TWD>
 struct FColCfg {
-    static constexpr int E = N >= 4096 ? 16 : (N >= 8 ? 8 : N);
+    // 1024- and 2048-point columns: 4 per lane, so that a single large
+    // system still puts several waves on every SIMD
+    static constexpr int E = N >= 4096 ? 16 : (N >= 1024 ? 4 : (N >= 8 ? 8 : N));
     static constexpr int T = N / E;
     static constexpr int THREADS = TWD * T;
     static constexpr int LL = lds_len(N);
