@@ -62,6 +62,7 @@ struct FusedBatch {
         int nc = 0, ny = 0;
         bool real = false;      // any instance has real fields (prologue)
         bool wide = false;      // row kernel for more than 3 real fields
+        bool cons = false;      // instances with stored constraint fields
         std::vector<int> insts;
         int lpb = 0, twd = 0;   // tile shapes
         std::vector<Tables> chunks;
@@ -478,13 +479,14 @@ static int build(FusedBatch &B)
             for (long h0 = 0; h0 < a.g.nh / 2; h0 += FUSED_ELEM_BLOCK) er.push_back(FusedBlock{(int)i, (int)h0});
             continue;
         }
-        auto key = std::make_pair(a.g.nc * 2 + (a.nreal > 3 ? 1 : 0), a.g.sy);
+        auto key = std::make_pair(a.g.nc * 4 + (a.nreal > 3 ? 2 : 0) + (a.ncons > 0 ? 1 : 0), a.g.sy);
         if (!cls.count(key)) {
             cls[key] = (int)B.classes.size();
             B.classes.emplace_back();
             B.classes.back().nc = a.g.nc;
             B.classes.back().ny = a.g.sy;
             B.classes.back().wide = a.nreal > 3;
+            B.classes.back().cons = a.ncons > 0;
         }
         FusedBatch::Class &c = B.classes[cls[key]];
         c.insts.push_back((int)i);
@@ -536,7 +538,7 @@ static int launch_prologue(FusedBatch &B)
     for (auto &c : B.classes)
         if (c.real)
             for (auto &t : c.chunks)
-                HIP_TRY(fused_cols_launch(c.ny, c.twd, B.d, t.cb, t.ncb, 0, 0, 0, B.stream));
+                HIP_TRY(fused_cols_launch(c.ny, c.twd, c.cons, B.d, t.cb, t.ncb, 0, 0, 0, B.stream));
     return CUPSS_OK;
 }
 
@@ -553,7 +555,7 @@ static int launch_class(FusedBatch &B, size_t ci, int n0, int k, int rel0, hipSt
             unsigned long long *ts = nullptr;
             if (timed && B.pcount < FusedBatch::PROF_SLOTS)
                 ts = B.pts + (size_t)(B.pcount++) * FUSED_TS_SLOTS * FUSED_TS_STRIDE;
-            HIP_TRY(fused_cols_launch(c.ny, c.twd, B.d, t.cb, t.ncb, n & 1, (uint32_t)(n - rel0), 1, st, ts));
+            HIP_TRY(fused_cols_launch(c.ny, c.twd, c.cons, B.d, t.cb, t.ncb, n & 1, (uint32_t)(n - rel0), 1, st, ts));
         }
     return CUPSS_OK;
 }
@@ -714,7 +716,7 @@ static int tune(FusedBatch &B)
             const int r = time_ms(
                 B, T, reps,
                 [&]() -> int {
-                    HIP_TRY(fused_cols_launch(c.ny, twd, B.d, t[0].cb, t[0].ncb, 0, 0, 1, B.stream));
+                    HIP_TRY(fused_cols_launch(c.ny, twd, c.cons, B.d, t[0].cb, t[0].ncb, 0, 0, 1, B.stream));
                     return CUPSS_OK;
                 },
                 &ms);
@@ -1049,8 +1051,9 @@ static std::string schedule_text(const FusedBatch &B)
     std::string out;
     char line[256];
     for (const auto &c : B.classes) {
-        std::snprintf(line, sizeof line, "class %dx%d%s: %zu instances, rows/wg %d, cols/wg %d, chunks %zu\n",
-                      2 * c.nc, c.ny, c.wide ? " (wide)" : "", c.insts.size(), c.lpb, c.twd, c.chunks.size());
+        std::snprintf(line, sizeof line, "class %dx%d%s%s: %zu instances, rows/wg %d, cols/wg %d, chunks %zu\n",
+                      2 * c.nc, c.ny, c.wide ? " (wide)" : "", c.cons ? " (stored fields)" : "", c.insts.size(),
+                      c.lpb, c.twd, c.chunks.size());
         out += line;
     }
     std::snprintf(line, sizeof line, "elementwise: %zu instances\nchunk target: %.0f MB%s\n", B.elem.size(),
@@ -1197,7 +1200,8 @@ static int kernel_time(FusedBatch &B, float *avg_ms, double *points, char *name,
     double pts = 0;
     for (int i : c.insts) pts += (double)B.h[i].g.n;
     if (points) *points = pts;
-    if (name && len > 0) std::snprintf(name, (size_t)len, "k_fcols<%d, %d>", c.ny, c.twd);
+    if (name && len > 0)
+        std::snprintf(name, (size_t)len, "k_fcols<%d, %d, %s>", c.ny, c.twd, c.cons ? "true" : "false");
     return CUPSS_OK;
 }
 

@@ -321,7 +321,9 @@ struct FColCfg {
     static constexpr bool valid() { return THREADS >= 16 && THREADS <= 512; }
 };
 
-template <int N, int TWD>
+// CONS: the instances may have stored constraint fields (compiled out
+// otherwise, which keeps the kernel within 128 VGPRs: 4 waves per SIMD)
+template <int N, int TWD, bool CONS>
 __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const FusedInst *__restrict__ insts,
                                                                    const FusedBlock *__restrict__ blocks,
                                                                    int parity, uint32_t step_rel, int mode,
@@ -411,7 +413,7 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
         }
         // ---- stored constraints (setNotDynamic_k, src/field_kernels.cu:130-197):
         // their values of step m+1, from the pre-step state of step m+1
-        for (int ci = 0; ci < a.ncons; ci++) {
+        for (int ci = 0; ci < (CONS ? a.ncons : 0); ci++) {
             const FusedTarget &TG = a.tgt[1 + ci];
             float2 v[E];
             target_sum<N, E, T>(a, TG, cb, cur, nw, xv, false, parity, active, col, L, t, tws, v);
@@ -456,7 +458,7 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
         // prologue: the real fields of the current state
 #pragma unroll
         for (int e = 0; e < E; e++) nw[e] = cur[e];
-        for (int ci = 0; ci < a.ncons; ci++) {
+        for (int ci = 0; ci < (CONS ? a.ncons : 0); ci++) {
             const int z = a.tgt[1 + ci].z;
             if (z < 0) continue;
             float2 v[E];
@@ -1103,31 +1105,37 @@ hipError_t fused_rows_launch(int nc, int lpb, bool wide, const FusedInst *inst, 
 }
 
 template <int N>
-static hipError_t fcols_n(int twd, const FusedInst *inst, const FusedBlock *blocks, int nblocks, int parity,
-                          uint32_t step_rel, int mode, hipStream_t s, const float2 *sets, unsigned long long *ts)
+static hipError_t fcols_n(int twd, bool cons, const FusedInst *inst, const FusedBlock *blocks, int nblocks,
+                          int parity, uint32_t step_rel, int mode, hipStream_t s, const float2 *sets,
+                          unsigned long long *ts)
 {
-#define X(TWD)                                                                                                \
-    if constexpr (FColCfg<N, TWD>::valid()) {                                                                \
-        if (twd == TWD) {                                                                                     \
-            hipLaunchKernelGGL((k_fcols<N, TWD>), dim3(nblocks), dim3(FColCfg<N, TWD>::THREADS), 0, s, inst, \
-                               blocks, parity, step_rel, mode, sets, ts);                                     \
-            return hipGetLastError();                                                                         \
-        }                                                                                                     \
+#define X(TWD)                                                                                                   \
+    if constexpr (FColCfg<N, TWD>::valid()) {                                                                   \
+        if (twd == TWD) {                                                                                        \
+            if (cons)                                                                                            \
+                hipLaunchKernelGGL((k_fcols<N, TWD, true>), dim3(nblocks), dim3(FColCfg<N, TWD>::THREADS), 0, s, \
+                                   inst, blocks, parity, step_rel, mode, sets, ts);                              \
+            else                                                                                                 \
+                hipLaunchKernelGGL((k_fcols<N, TWD, false>), dim3(nblocks), dim3(FColCfg<N, TWD>::THREADS), 0,   \
+                                   s, inst, blocks, parity, step_rel, mode, sets, ts);                           \
+            return hipGetLastError();                                                                            \
+        }                                                                                                        \
     }
     FUSED_TWD(X)
 #undef X
     return hipErrorInvalidValue;
 }
 
-hipError_t fused_cols_launch(int ny, int twd, const FusedInst *inst, const FusedBlock *blocks, int nblocks,
-                             int parity, uint32_t step_rel, int mode, hipStream_t s, unsigned long long *ts)
+hipError_t fused_cols_launch(int ny, int twd, bool cons, const FusedInst *inst, const FusedBlock *blocks,
+                             int nblocks, int parity, uint32_t step_rel, int mode, hipStream_t s,
+                             unsigned long long *ts)
 {
     const float2 *sets = twiddle_sets(s);
     if (!sets) return hipErrorOutOfMemory;
     switch (ny) {
 #define X(N) \
     case N:  \
-        return fcols_n<N>(twd, inst, blocks, nblocks, parity, step_rel, mode, s, sets, ts);
+        return fcols_n<N>(twd, cons, inst, blocks, nblocks, parity, step_rel, mode, s, sets, ts);
         FUSED_COL_CASES(X)
 #undef X
     default:

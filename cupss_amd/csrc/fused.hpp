@@ -131,7 +131,8 @@ hipError_t fused_rows_launch(int nc, int lpb, bool wide, const FusedInst *inst, 
 // (ts: null, or FUSED_TS_SLOTS pairs {earliest start, latest end} of
 // wall-clock ticks, FUSED_TS_STRIDE apart, that the launch's workgroups
 // min/max-update; the launch spans min(starts) .. max(ends))
-hipError_t fused_cols_launch(int ny, int twd, const FusedInst *inst, const FusedBlock *blocks, int nblocks,
+// (cons: some instance of the launch has stored constraint fields)
+hipError_t fused_cols_launch(int ny, int twd, bool cons, const FusedInst *inst, const FusedBlock *blocks, int nblocks,
                              int parity, uint32_t step_rel, int mode, hipStream_t s, unsigned long long *ts = nullptr);
 // the slots of n timed launches to {~0, 0} (a kernel, so that graphs can hold it)
 hipError_t fused_ts_reset_launch(unsigned long long *ts, int n, hipStream_t s);
