@@ -598,8 +598,20 @@ __global__ __launch_bounds__(FUSED_ELEM_BLOCK) void k_felem(const FusedInst *__r
     const int hy = g.sy / 2;  // sy >= 8 for fused instances
     const long pidx = (long)blk.line + threadIdx.x;
     if (pidx >= g.nh / 2) return;  // no barriers below
-    const int kx = (int)(pidx % g.nc1);
-    const long r = pidx / g.nc1;
+    // Pairs of the interior columns 1..nc-1 first, then those of the kx = 0
+    // and kx = nc planes, whose noise takes two Philox calls (Hermitian
+    // partners): only the last waves take that branch, none diverges on it.
+    const long prow = (long)hy * g.sz, nint = (long)(g.nc - 1) * prow;
+    int kx;
+    long r;
+    if (pidx < nint) {
+        kx = 1 + (int)(pidx % (g.nc - 1));
+        r = pidx / (g.nc - 1);
+    } else {
+        const long q = pidx - nint;
+        kx = (q & 1) ? g.nc : 0;
+        r = q >> 1;
+    }
     const int j = (int)(r % hy), kz = (int)(r / hy);
     const UpdateArgs &u = a.u;
     const bool noisy = u.noisy && a.noise_mode == NOISE_SPECTRAL;
