@@ -275,24 +275,47 @@ def cpu_baseline_ensemble(members, budget_s=20.0):
     one EW and one KPZ instance of each size, a few steps each; the
     ensemble's aggregate rate is extrapolated from the per-instance step
     times (the reference advances one instance at a time)."""
-    from oracle import oracle as O
-    per = {}
-    for n in (64, 128, 256, 512):
-        for kpz in (False, True):
-            lam = 1.0 if kpz else 0.0
-            s, h = kpz_system(O.System, O.pres, n, 1.0, lam, precision="f32")
-            s.step(1)
-            k = max(1, int(budget_s / 8 / (n * n * (2.0e-6 if kpz else 0.7e-6))))
-            k = min(k, 200)
-            t0 = time.perf_counter()
-            s.step(k)
-            per[(n, kpz)] = (time.perf_counter() - t0) / k
+    per = _cpu_sample(budget_s)
     total_t = sum(per[(n, lam != 0.0)] for n, _, lam in members)
     npts = sum(n * n for n, _, _ in members)
     return {"value": npts / total_t, "unit": "grid-point-steps/s", "cores": 1, "kind": "port",
             "sample": "one EW and one KPZ instance per size (64..512), a few steps each, oracle FP32 "
                       "(C2C, radix-2); ensemble rate = sum of points / sum of per-instance step times, %s" %
                       _cpu_name()}
+
+
+def _cpu_sample(budget_s):
+    """Per-instance step times of the oracle on one EW and one KPZ instance of
+    each ensemble size (a worker of the all-cores baseline: a spawned process
+    that loads only the oracle, never the GPU library)."""
+    from oracle import oracle as O
+    per = {}
+    for n in (64, 128, 256, 512):
+        for kpz in (False, True):
+            s, h = kpz_system(O.System, O.pres, n, 1.0, 1.0 if kpz else 0.0, precision="f32")
+            s.step(1)
+            k = min(200, max(1, int(budget_s / 8 / (n * n * (2.0e-6 if kpz else 0.7e-6)))))
+            t0 = time.perf_counter()
+            s.step(k)
+            per[(n, kpz)] = (time.perf_counter() - t0) / k
+    return per
+
+
+def cpu_baseline_ensemble_all_cores(members, budget_s=8.0):
+    """The same oracle sample on every core at once (instances dealt
+    round-robin over W worker processes): per-instance times measured while
+    all W workers run, ensemble time = their sum over the instances / W.
+    Runs before the GPU is touched, from spawned processes."""
+    import multiprocessing as mp
+    workers = max(1, min(16, os.cpu_count() or 1))
+    with mp.get_context("spawn").Pool(workers) as pool:
+        samples = pool.map(_cpu_sample, [budget_s] * workers)
+    per = {key: sum(p[key] for p in samples) / workers for key in samples[0]}
+    total_t = sum(per[(n, lam != 0.0)] for n, _, lam in members) / workers
+    npts = sum(n * n for n, _, _ in members)
+    return {"value": npts / total_t, "unit": "grid-point-steps/s", "cores": workers, "kind": "port",
+            "sample": "the 1-core sample run by %d processes at once, instances dealt round-robin, %s" %
+                      (workers, _cpu_name())}
 
 
 def _cpu_name():
@@ -349,6 +372,12 @@ def main():
               "multi-GPU sharding is a design note only.", file=sys.stderr)
         sys.exit(2)
 
+    # the all-cores CPU sample spawns worker processes: before this process
+    # touches the GPU
+    all_cores = None
+    if args.workload == "ensemble" and not args.no_cpu:
+        all_cores = cpu_baseline_ensemble_all_cores(ensemble_members())
+
     import cupss_amd as C
     wl = WORKLOADS[args.workload]
     if args.copy_calibration > 0:
@@ -384,6 +413,7 @@ def main():
                                     "ensemble")}
         if not args.no_cpu:
             out["cpu_baseline"] = cpu_baseline_ensemble(ensemble_members())
+            out["cpu_baseline_all_cores"] = all_cores
         print(json.dumps(out), flush=True)
         return
     dims = wl["dims"]
