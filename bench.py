@@ -294,7 +294,7 @@ def _cpu_sample(budget_s):
         for kpz in (False, True):
             s, h = kpz_system(O.System, O.pres, n, 1.0, 1.0 if kpz else 0.0, precision="f32")
             s.step(1)
-            k = min(200, max(1, int(budget_s / 8 / (n * n * (2.0e-6 if kpz else 0.7e-6)))))
+            k = min(200, max(1, int(budget_s / 8 / (n * n * (6.0e-7 if kpz else 2.5e-7)))))
             t0 = time.perf_counter()
             s.step(k)
             per[(n, kpz)] = (time.perf_counter() - t0) / k
