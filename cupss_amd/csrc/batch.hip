@@ -111,6 +111,7 @@ struct FusedBatch {
 
     long steps_run = 0;
     std::vector<uint64_t> gens;   // system generations the tables were built for
+    std::vector<uint64_t> vgens;  // prefactor-value generations the descriptors hold
 
     // 3D path (the batch of one of a 3D system): h[0] is the descriptor on
     // the device, twd3 the z-line tile width
@@ -504,7 +505,11 @@ static int build(FusedBatch &B)
     if (B.profile)
         if (int r = retable(B)) return r;
     B.gens.clear();
-    for (cupss_system s : B.sys) B.gens.push_back(s->gen);
+    B.vgens.clear();
+    for (cupss_system s : B.sys) {
+        B.gens.push_back(s->gen);
+        B.vgens.push_back(s->vgen);
+    }
     B.built = true;
     return CUPSS_OK;
 }
@@ -524,7 +529,13 @@ static bool stale_tables(const FusedBatch &B)
 static int refresh(FusedBatch &B)
 {
     if (B.staged) HIP_TRY(hipEventSynchronize(B.staged));
-    for (size_t i = 0; i < B.sys.size(); i++) run_view(B.sys[i], B.is_elem[i], B.h[i]);
+    for (size_t i = 0; i < B.sys.size(); i++) {
+        if (B.vgens[i] != B.sys[i]->vgen) {  // new prefactor values, same structure: new pool
+            if (int r = describe(B.sys[i], B.h[i])) return r;
+            B.vgens[i] = B.sys[i]->vgen;
+        }
+        run_view(B.sys[i], B.is_elem[i], B.h[i]);
+    }
     std::memcpy(B.pinned, B.h.data(), sizeof(FusedInst) * B.h.size());
     HIP_TRY(hipMemcpyAsync(B.d, B.pinned, sizeof(FusedInst) * B.h.size(), hipMemcpyHostToDevice, B.stream));
     if (!B.staged) HIP_TRY(hipEventCreateWithFlags(&B.staged, hipEventDisableTiming));

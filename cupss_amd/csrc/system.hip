@@ -160,12 +160,27 @@ extern "C" int cupss_field_add_implicit(cupss_system s, int field, cupss_pres p)
     return CUPSS_OK;
 }
 
+static void plan_fused(cupss_system s);
+static void parameters_changed(cupss_system s, bool same_shape);
+
+// same length and the same exponents in every monomial (values may differ)
+static bool same_shape(const std::vector<cupss_pres> &a, const cupss_pres *b, int n)
+{
+    if ((int)a.size() != n) return false;
+    for (int i = 0; i < n; i++)
+        if (a[i].q2n != b[i].q2n || a[i].iqx != b[i].iqx || a[i].iqy != b[i].iqy || a[i].iqz != b[i].iqz ||
+            a[i].invq != b[i].invq)
+            return false;
+    return true;
+}
+
 extern "C" int cupss_field_set_implicits(cupss_system s, int field, const cupss_pres *p, int n)
 {
     if (int r = check_field(s, field)) return r;
     if (n < 0 || n > MAX_IMP || (n > 0 && !p)) return fail(CUPSS_ERR_INVALID, "bad implicit list");
+    const bool shape = same_shape(s->f[field].imp, p, n);
     s->f[field].imp.assign(p, p + n);
-    s->gen++;
+    parameters_changed(s, shape);
     return CUPSS_OK;
 }
 
@@ -206,8 +221,9 @@ extern "C" int cupss_term_set_prefactors(cupss_system s, int term, const cupss_p
     TermRec &T = s->t[term];
     if (field_pres_count(s, T.field, npres - (int)T.pres.size()) > MAX_PRES)
         return fail(CUPSS_ERR_UNSUPPORTED, "too many prefactors");
+    const bool shape = same_shape(T.pres, pres, npres);
     T.pres.assign(pres, pres + npres);
-    s->gen++;
+    parameters_changed(s, shape);
     return CUPSS_OK;
 }
 
@@ -249,8 +265,6 @@ extern "C" int cupss_field_set_device_hooks(cupss_system s, int field, cupss_rea
     s->gen++;
     return CUPSS_OK;
 }
-
-static void plan_fused(cupss_system s);
 
 extern "C" int cupss_field_set_real(cupss_system s, int field, const float *host)
 {
@@ -679,7 +693,8 @@ static int general_advance(cupss_system s, int nsteps)
     if (K < 2 || nsteps < K || noisy || hooks_present(s)) return general_steps(s, nsteps);
     for (int i = 0; i < (int)s->f.size(); i++)
         if (int r = materialize(s, i)) return r;
-    if (s->gexec && (s->graph_gen != s->gen || s->graph_stream != s->stream)) drop_general_graph(s);
+    if (s->gexec && (s->graph_gen != s->gen || s->graph_vgen != s->vgen || s->graph_stream != s->stream))
+        drop_general_graph(s);
     int done = 0;
     if (s->gexec && !graph_parity(s)) {
         if (int r = general_steps(s, 1)) return r;
@@ -700,6 +715,7 @@ static int general_advance(cupss_system s, int nsteps)
         s->graph = g;
         HIP_TRY(hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0));
         s->graph_gen = s->gen;
+        s->graph_vgen = s->vgen;
         s->graph_stream = s->stream;
         HIP_TRY(hipGraphLaunch(s->gexec, s->stream));
         done = K;
@@ -943,6 +959,41 @@ static void plan_fused(cupss_system s)
     if (pool > (size_t)FUSED_MAX_POOL) return;
     P.phi = phi;
     P.ok = true;
+}
+
+static bool same_plan_shape(const FusedPlan &a, const FusedPlan &b)
+{
+    if (a.ok != b.ok || a.phi != b.phi || a.real != b.real || a.derived != b.derived || a.cons != b.cons ||
+        a.groups.size() != b.groups.size())
+        return false;
+    for (size_t i = 0; i < a.groups.size(); i++) {
+        const FusedGroup &x = a.groups[i], &y = b.groups[i];
+        if (x.target != y.target || x.terms != y.terms || x.coef.size() != y.coef.size() ||
+            !same_shape(x.pres, y.pres.data(), (int)y.pres.size()))
+            return false;
+    }
+    return true;
+}
+
+// evolver::updateParameter (src/evolver.cpp:386-394) through new prefactor
+// values: the fused plan is made again (its groups hold the product terms'
+// prefactors and folded coefficients).  When the monomials keep their shape
+// and the plan its structure, only values changed: vgen moves and the
+// batches and graphs refresh their descriptors without re-planning tables,
+// re-tuning or re-classifying; otherwise gen moves as for any program change.
+static void parameters_changed(cupss_system s, bool same)
+{
+    s->cont_step = ~0ull;  // real fields left by a run used the old multipliers
+    if (!s->prepared) {
+        s->gen++;
+        return;
+    }
+    FusedPlan old = s->plan;
+    plan_fused(s);
+    if (same && same_plan_shape(old, s->plan))
+        s->vgen++;
+    else
+        s->gen++;
 }
 
 extern "C" int cupss_system_time_advance(cupss_system s, int nsteps, float *ms)

@@ -110,6 +110,7 @@ struct cupss_system_t {
     int device = 0;
     void *batch = nullptr;    // internal one-instance fused batch
     uint64_t gen = 1;         // bumped by every change that alters the step program
+    uint64_t vgen = 1;        // bumped by prefactor value changes that keep the program's structure
     // general path: K steps captured once in a hipGraph and replayed
     // (noiseless systems without hooks; K even so every field's spectrum
     // returns to its first buffer)
@@ -117,6 +118,7 @@ struct cupss_system_t {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     uint64_t graph_gen = 0;
+    uint64_t graph_vgen = 0;
     hipStream_t graph_stream = nullptr;
     std::vector<int> graph_cur;  // each field's buffer parity the graph was captured at
 };
