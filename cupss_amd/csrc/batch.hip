@@ -543,13 +543,22 @@ static int refresh(FusedBatch &B)
     return CUPSS_OK;
 }
 
+// Launches are skipped where every instance they serve resumes (its
+// workgroups would only return): the one-step-per-call loop then costs the
+// steps' own launches.
 static int launch_prologue(FusedBatch &B)
 {
-    if (B.ntb > 0) HIP_TRY(fused_transpose_in_launch(B.d, B.tb, B.ntb, B.stream));
-    for (auto &c : B.classes)
-        if (c.real)
+    bool all = true;
+    for (size_t i = 0; i < B.h.size(); i++)
+        if (!B.is_elem[i] && !B.h[i].resume) all = false;
+    if (B.ntb > 0 && !all) HIP_TRY(fused_transpose_in_launch(B.d, B.tb, B.ntb, B.stream));
+    for (auto &c : B.classes) {
+        bool resumed = true;
+        for (int i : c.insts) resumed = resumed && B.h[i].resume;
+        if (c.real && !resumed)
             for (auto &t : c.chunks)
                 HIP_TRY(fused_cols_launch(c.ny, c.twd, c.cons, B.d, t.cb, t.ncb, 0, 0, 0, B.stream));
+    }
     return CUPSS_OK;
 }
 
