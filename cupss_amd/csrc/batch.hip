@@ -1022,6 +1022,8 @@ static int run3d(FusedBatch &B, cupss_system s, int nsteps)
         if (B.profile && B.pcount < FusedBatch::PROF_SLOTS)
             ts = B.pts + (size_t)(B.pcount++) * FUSED_TS_SLOTS * FUSED_TS_STRIDE;
         HIP_TRY(fused3d_z_launch(g.sz, B.twd3, B.d, g.nc1, g.sy, par0 ^ (i & 1), rel0 + (uint32_t)i, 1, st, ts));
+        // the kx = 0, sx/2 planes of the new state: partners are in other z lines
+        HIP_TRY(hermitian_planes(g, F.spec[par0 ^ (i & 1) ^ 1], st));
         for (int f = 0; f < a.nreal; f++) HIP_TRY(lines_y(g, +1, s->fw[f], s->fz[f], a.kmax[f] + 1, st));
     }
     if (a.nder > 0) HIP_TRY(fused3d_derived_launch(B.d, g.nh, par0 ^ ((nsteps - 1) & 1), st));

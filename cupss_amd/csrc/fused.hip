@@ -310,6 +310,31 @@ __device__ __forceinline__ void z_out(const FusedInst &a, int f, const Geom &g, 
     __syncthreads();
 }
 
+// Hermitian projection of the columns kx = 0 and kx = sx/2 (2D): bin ky
+// and its partner N - ky of such a column both become their Hermitian mean,
+// as the reference's IFFT / real part / FFT round trip of every field every
+// step makes them (src/field.cpp:59-89; the interior columns stand for their
+// partners implicitly).  Without it the anti-Hermitian part of these columns
+// -- invisible to C2R, fed by rounding -- grows in linearly unstable bands
+// until it leaks into the real fields.  v[e] holds bin t + e*T of the lane's
+// column; the whole workgroup calls this (barriers), only `edge` lines
+// change.
+template <int N, int E, int T>
+__device__ __forceinline__ void hermitian_col(float2 (&v)[E], float2 *L, int t, bool edge)
+{
+#pragma unroll
+    for (int e = 0; e < E; e++) L[lpad(t + e * T)] = v[e];
+    __syncthreads();
+    if (edge) {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const float2 p = L[lpad((N - (t + e * T)) & (N - 1))];
+            v[e] = make_float2(0.5f * (v[e].x + p.x), 0.5f * (v[e].y - p.y));
+        }
+    }
+    __syncthreads();
+}
+
 template <int N, int TWD>
 struct FColCfg {
     // 1024- and 2048-point columns: 4 per lane, so that a single large
@@ -406,6 +431,10 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
             }
             nw[e] = s;
         }
+        // workgroups holding the column kx = 0 or kx = sx/2 (uniform test)
+        const bool edge_wg = blk.line == 0 || blk.line + TWD > g.nc;
+        const bool edge = active && (kx == 0 || kx == g.nc);
+        if (edge_wg) hermitian_col<N, E, T>(nw, L, t, edge);
         if (active) {
             float2 *__restrict__ sout = a.spec[parity ^ 1];
 #pragma unroll
@@ -447,6 +476,7 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
                         v[e].y *= rl;
                     }
             }
+            if (edge_wg) hermitian_col<N, E, T>(v, L, t, edge);
             if (active) {
                 float2 *__restrict__ cout = a.cs[ci][parity ^ 1];
 #pragma unroll

@@ -95,7 +95,12 @@ hipError_t r2c(const Geom &g, const R2CSource &src, float2 *out, hipStream_t s);
 // real out[n] <- scale * C2R(mask_p(in)); scratch[nh] is clobbered (may not alias in)
 hipError_t c2r(const Geom &g, const float2 *in, int mask_p, float scale, float *out, float2 *scratch,
                hipStream_t s);
+// the update of one field, then the Hermitian projection of its kx = 0 and
+// kx = sx/2 planes (hermitian_planes)
 hipError_t update(const UpdateArgs &a, hipStream_t s);
+// bins (kx, ky, kz), (kx, -ky, -kz) of the planes kx = 0, sx/2 -> their
+// Hermitian mean, in place
+hipError_t hermitian_planes(const Geom &g, float2 *spec, hipStream_t s);
 // unnormalised C2C transforms (dir -1 forward, +1 inverse) of the lines along
 // y of a natural half spectrum with kx < ncols: line (kx, kz) holds
 // in[(kz*sy + ky)*nc1 + kx], ky < sy; out may be in

@@ -469,12 +469,47 @@ hipError_t spectral_noise_fill(const Geom &g, float2 *eta, uint64_t seed, uint64
     return hipGetLastError();
 }
 
+// Hermitian projection of the kx = 0 and kx = sx/2 planes of a half
+// spectrum: bin (kx, ky, kz) and its partner (kx, -ky, -kz) both become
+// their Hermitian mean (a self-partnered bin keeps its real part).  The
+// interior planes stand for their partners implicitly; on these two planes
+// both partners are stored, and the reference's IFFT / real part / FFT round
+// trip of every field every step (src/field.cpp:59-89) keeps them conjugate.
+// Without it the anti-Hermitian part -- invisible to C2R, fed by rounding in
+// the transforms -- grows unchecked in linearly unstable bands (e.g. the
+// spinodal band of Cahn-Hilliard, x12 per 1000 steps at dt = 0.01) until it
+// leaks into the real fields.  One thread per bin of the two planes.
+__global__ __launch_bounds__(256) void k_hermitian_planes(float2 *__restrict__ h, Geom g)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long per = (long)g.sy * g.sz;
+    if (i >= 2 * per) return;
+    const int kx = i < per ? 0 : g.nc;
+    const long r = i % per;
+    const int ky = (int)(r % g.sy), kz = (int)(r / g.sy);
+    const int py = (g.sy - ky) % g.sy, pz = (g.sz - kz) % g.sz;
+    if (pz < kz || (pz == kz && py < ky)) return;  // the partner's thread writes both
+    const long a = ((long)kz * g.sy + ky) * g.nc1 + kx, b = ((long)pz * g.sy + py) * g.nc1 + kx;
+    const float2 u = h[a], v = h[b];
+    const float2 m = make_float2(0.5f * (u.x + v.x), 0.5f * (u.y - v.y));
+    h[a] = m;
+    if (b != a) h[b] = make_float2(m.x, -m.y);
+}
+
+hipError_t hermitian_planes(const Geom &g, float2 *spec, hipStream_t s)
+{
+    const long n = 2L * g.sy * g.sz;
+    hipLaunchKernelGGL(k_hermitian_planes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, spec, g);
+    return hipGetLastError();
+}
+
 hipError_t update(const UpdateArgs &a, hipStream_t s)
 {
     const unsigned rows = (unsigned)((long)a.g.sy * a.g.sz);
     const unsigned threads = a.g.nc1 >= 256 ? 256 : (a.g.nc1 >= 128 ? 128 : 64);
     hipLaunchKernelGGL(k_update, dim3(rows), dim3(threads), 0, s, a);
-    return hipGetLastError();
+    if (hipError_t e = hipGetLastError()) return e;
+    return hermitian_planes(a.g, a.out, s);
 }
 
 }  // namespace cupss

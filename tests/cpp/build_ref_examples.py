@@ -4,8 +4,8 @@ tests/test_gpu_ref_examples.py, which run them on an MI355X.
 
 The sources are read from the reference tree (/root/reference/examples) and
 compiled from a temporary copy outside this repository (its inc/ pointing at
-include/; unused <cuda.h>/<cuda_runtime.h> includes would be dropped, as
-tests/test_examples_compile.py does); only the binaries land in
+include/; the unused <cuda.h>/<cuda_runtime.h> includes of example 04 are
+dropped, as tests/test_examples_compile.py does); only the binaries land in
 tests/cpp/bin/ref_examples/, which git ignores and the GPU runs receive with
 the tree.  Without the reference tree this does nothing.
 
@@ -26,7 +26,8 @@ LIBDIR = os.path.join(ROOT, "cupss_amd")
 # (source under examples/, binary name)
 PROGRAMS = [("01_diffusion/01_diffusion.cpp", "01_diffusion"),
             ("05_edwards_wilkinson/05_edwards_wilkinson.cpp", "05_edwards_wilkinson"),
-            ("07_inhomogeneous_diffusion/07_inhomogeneous_diffusion.cu", "07_inhomogeneous_diffusion")]
+            ("07_inhomogeneous_diffusion/07_inhomogeneous_diffusion.cu", "07_inhomogeneous_diffusion"),
+            ("04_model_h/modelh_explicit_velocity.cpp", "modelh_explicit_velocity")]
 
 
 def build(verbose=False):
