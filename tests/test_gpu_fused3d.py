@@ -136,3 +136,38 @@ def test_cahn_hilliard_256_cubed_mass_and_general():
     print("  mean %.9e -> %.9e, fused-vs-general %.3e" % (m0, m1, err), flush=True)
     assert abs(m1 - m0) <= 1e-6 * max(1.0, float(np.sqrt(np.mean(x.astype(np.float64) ** 2))))
     assert err <= 1e-5
+
+
+def spec_mixed_terms(dims, seed=13):
+    """One dynamic field with every kind of term the fused steps take:
+    a constant (SRC_CONST), a derived field (SRC_DERIVED: psi = iqx*phi),
+    phi itself, and two product groups with different multipliers (one of
+    them reading a second derived field chi = q^2*phi).
+    dt phi + 0.5*q^2*phi = 0.3 + 0.4*psi - 0.2*q^2*phi^3 + 0.1*iqy*(phi*chi)"""
+    n = dims[0] * dims[1] * dims[2]
+    x0 = np.random.default_rng(seed).uniform(-0.2, 0.2, n).astype(np.float32)
+    terms = [("phi", [], [(0.3,)]),
+             ("phi", ["psi"], [(0.4,)]),
+             ("phi", ["phi", "phi", "phi"], [(-0.2, 1)]),
+             ("phi", ["phi", "chi"], [(0.1, 0, 0, 1)]),
+             ("psi", ["phi"], [(1.0, 0, 1)]),
+             ("chi", ["phi"], [(1.0, 1)])]
+    return {"dims": dims, "dt": 0.01, "fields": [("phi", True), ("psi", False), ("chi", False)],
+            "implicit": {"phi": [(-0.5, 1)]}, "terms": terms, "init": {"phi": x0}}
+
+
+@pytest.mark.parametrize("dims", [(64, 32, 1), (16, 16, 32)])
+def test_mixed_terms_fused_vs_oracle(dims):
+    """The fused 2D and 3D steps on a system with constant, derived and
+    grouped product terms, against the oracle's FP32 scheme."""
+    spec = spec_mixed_terms(dims)
+    g, gid = gpu(spec, profile=True)
+    o, oid = build_system(O.System, O.pres, dict(spec, precision="f32"))
+    g.step(50)
+    o.step(50)
+    name = g.kernel_time()[2]
+    assert name.startswith("k_f3z<" if dims[2] > 1 else "k_fcols<"), name
+    for f in ("phi", "psi", "chi"):
+        err = rel_l2(g.get_real(gid[f]), o.get_real(oid[f]))
+        print("  %s %-4s gpu-vs-f32 %.3e" % (dims, f, err), flush=True)
+        assert err <= 1e-4, f
