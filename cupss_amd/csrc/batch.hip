@@ -62,7 +62,7 @@ struct FusedBatch {
         int nc = 0, ny = 0;
         bool real = false;      // any instance has real fields (prologue)
         bool wide = false;      // row kernel for more than 3 real fields
-        bool cons = false;      // instances with stored constraint fields
+        int cons = 0;           // stored fields of the instances: 1 = constraints, 2 = dynamic fields too
         std::vector<int> insts;
         int lpb = 0, twd = 0;   // tile shapes
         std::vector<Tables> chunks;
@@ -223,6 +223,18 @@ static int pool_term(FusedInst &a, int &np, const std::vector<cupss_pres> &pres,
     return CUPSS_OK;
 }
 
+// field f is a dynamic field of the plan whose dealiased copy was never
+// written: it reads as zero in the products (the reference's step 0)
+static bool step0_zero(cupss_system s, int f)
+{
+    const FusedPlan &P = s->plan;
+    if (s->f[f].dstate != 0) return false;
+    if (f == P.phi) return true;
+    for (int k = 0; k < P.nxdyn; k++)
+        if (P.cons[k] == f) return true;
+    return false;
+}
+
 // device descriptor of system s (its fused plan must be ok); natural: the
 // 3D path, which needs no column-major spectra
 static int describe(cupss_system s, FusedInst &a, bool natural = false)
@@ -248,9 +260,13 @@ static int describe(cupss_system s, FusedInst &a, bool natural = false)
     int np = 0;
     auto index_of = [](const std::vector<int> &v, int x) { return (int)(std::find(v.begin(), v.end(), x) - v.begin()); };
     // derived fields: one multiplier each (also L_f of their real fields)
+    if (!fused_noise_ok(s))
+        return fail(CUPSS_ERR_UNSUPPORTED, "the fused step draws the noise of a second dynamic field in spectral space only");
     a.nder = (int)P.derived.size();
-    for (size_t d = 0; d < P.derived.size(); d++)
+    for (size_t d = 0; d < P.derived.size(); d++) {
         if (int r = pool_term(a, np, s->t[s->f[P.derived[d]].terms[0]].pres, a.dmul[d])) return r;
+        a.dbase[d] = P.dbase[d] == P.phi ? -1 : index_of(P.cons, P.dbase[d]);
+    }
     // stored constraint fields
     a.ncons = (int)P.cons.size();
     // targets: phi, then the stored fields; groups are ordered the same way
@@ -286,21 +302,41 @@ static int describe(cupss_system s, FusedInst &a, bool natural = false)
             FT.pres0 = R.pres0;
             FT.npres = R.npres;
             FT.idx = 0;
+            // a dynamic target (phi, or a stored dynamic field) reads the
+            // values before the step, a constraint the dynamic fields' after it
+            const bool dyn_target = k == 0 || (int)k - 1 < P.nxdyn;
             if (T.fac.empty()) {
                 FT.src = SRC_CONST;
             } else if (T.fac[0] == P.phi) {
-                FT.src = k == 0 ? SRC_OLD : SRC_NEW;
+                FT.src = dyn_target ? SRC_OLD : SRC_NEW;
             } else if (std::find(P.derived.begin(), P.derived.end(), T.fac[0]) != P.derived.end()) {
                 FT.src = SRC_DERIVED;
                 FT.idx = index_of(P.derived, T.fac[0]);
             } else {
-                FT.src = SRC_CONS;
                 FT.idx = index_of(P.cons, T.fac[0]);
+                FT.src = (!dyn_target && FT.idx < P.nxdyn) ? SRC_DYN_NEW : SRC_CONS;
             }
         }
         TG.nimp = 0;
         TG.imp0 = np;
         TG.z = -1;
+        TG.dyn = 0;
+        TG.noisy = 0;
+        if (k > 0 && (int)k - 1 < P.nxdyn) {
+            // a further dynamic field: its noise (spectral) and D(q) from the
+            // implicit prefactors below
+            UpdateArgs U;
+            build_update(s, f, U);
+            const FieldRec &G = s->f[f];
+            TG.dyn = 1;
+            TG.noisy = G.noisy ? 1 : 0;
+            TG.noise_base = U.noise_base;
+            TG.noise_q2n = U.noise_q2n;
+            TG.noise_invq = U.noise_invq;
+            TG.key0 = (uint32_t)G.seed;
+            TG.key1 = (uint32_t)(G.seed >> 32);
+            TG.stream = (uint32_t)f;
+        }
         if (k > 0) {
             for (const cupss_pres &q : s->f[f].imp) {
                 if (np >= FUSED_MAX_POOL) return fail(CUPSS_ERR_UNSUPPORTED, "too many prefactors for the fused step");
@@ -329,12 +365,14 @@ static int describe(cupss_system s, FusedInst &a, bool natural = false)
         a.kmax[i] = a.p[i] > 0 ? std::min(g.nc, g.sx / (a.p[i] + 1)) : g.nc;
         a.kymax[i] = a.p[i] > 0 ? g.sy / (a.p[i] + 1) : g.sy;
         a.Z[i] = s->fz[i];
-        a.zero[i] = (f == P.phi && F.dstate == 0) ? 1 : 0;
+        a.zero[i] = step0_zero(s, f) ? 1 : 0;
         a.zsrc[i] = -1;
+        a.lbase[i] = -1;
         a.lmul[i] = TermRefDev{nullptr, 0, 0, 0, 0};
         const int d = index_of(P.derived, f), c = index_of(P.cons, f);
         if (d < (int)P.derived.size()) {
             a.lmul[i] = a.dmul[d];
+            a.lbase[i] = a.dbase[d];
         } else if (c < (int)P.cons.size()) {
             a.zsrc[i] = c;
             a.tgt[1 + c].z = (int)i;
@@ -371,7 +409,7 @@ static void run_view(cupss_system s, bool elementwise, FusedInst &a)
         a.cs[c][0] = s->fc[c][b0];
         a.cs[c][1] = s->fc[c][b0 ^ 1];
     }
-    for (int k = 0; k < a.nreal; k++) a.zero[k] = (s->plan.real[k] == s->plan.phi && F.dstate == 0) ? 1 : 0;
+    for (int k = 0; k < a.nreal; k++) a.zero[k] = step0_zero(s, s->plan.real[k]) ? 1 : 0;
     for (int d = 0; d < a.nder; d++) {
         FieldRec &D = s->f[s->plan.derived[d]];
         a.dout[d] = D.spec[D.cur];
@@ -480,14 +518,17 @@ static int build(FusedBatch &B)
             for (long h0 = 0; h0 < a.g.nh / 2; h0 += FUSED_ELEM_BLOCK) er.push_back(FusedBlock{(int)i, (int)h0});
             continue;
         }
-        auto key = std::make_pair(a.g.nc * 4 + (a.nreal > 3 ? 2 : 0) + (a.ncons > 0 ? 1 : 0), a.g.sy);
+        int stored = a.ncons > 0 ? 1 : 0;
+        for (int k = 0; k < a.ncons; k++)
+            if (a.tgt[1 + k].dyn) stored = 2;
+        auto key = std::make_pair(a.g.nc * 8 + (a.nreal > 3 ? 4 : 0) + stored, a.g.sy);
         if (!cls.count(key)) {
             cls[key] = (int)B.classes.size();
             B.classes.emplace_back();
             B.classes.back().nc = a.g.nc;
             B.classes.back().ny = a.g.sy;
             B.classes.back().wide = a.nreal > 3;
-            B.classes.back().cons = a.ncons > 0;
+            B.classes.back().cons = stored;
         }
         FusedBatch::Class &c = B.classes[cls[key]];
         c.insts.push_back((int)i);
@@ -847,7 +888,7 @@ static int run(FusedBatch &B, int nsteps)
     for (cupss_system s : B.sys) {
         s->resume_now = !will_tune && s->cont_step == s->step && s->cont_gen == s->gen;
         s->cont_now = !s->plan.cons.empty() && s->resume_now;
-        if (!s->plan.cons.empty() && !s->cont_now) {
+        if ((int)s->plan.cons.size() > s->plan.nxdyn && !s->cont_now) {
             if (int r = general_constraint_phase(s)) return r;
             general = true;
         }
@@ -1123,7 +1164,9 @@ extern "C" int cupss_batch_add(cupss_batch b, cupss_system s)
     for (auto &F : s->f)
         if (F.real_hook || F.spec_hook || F.real_dev || F.spec_dev)
             return fail(CUPSS_ERR_UNSUPPORTED, "systems with callbacks cannot be batched");
-    if (s->f[s->plan.phi].dstate == 3) {
+    if (!fused_noise_ok(s))
+        return fail(CUPSS_ERR_UNSUPPORTED, "the batched step draws the noise of a second dynamic field in spectral space only");
+    if (fused_stale(s)) {
         // consume a stale dealiased copy with one general step, as a lone system would
         if (int r = general_steps(s, 1)) return r;
         HIP_TRY(hipStreamSynchronize(s->stream));
@@ -1223,7 +1266,8 @@ static int kernel_time(FusedBatch &B, float *avg_ms, double *points, char *name,
     for (int i : c.insts) pts += (double)B.h[i].g.n;
     if (points) *points = pts;
     if (name && len > 0)
-        std::snprintf(name, (size_t)len, "k_fcols<%d, %d, %s>", c.ny, c.twd, c.cons ? "true" : "false");
+        std::snprintf(name, (size_t)len, "k_fcols<%d, %d, %s, %s>", c.ny, c.twd, c.cons ? "true" : "false",
+                      c.cons == 2 ? "true" : "false");
     return CUPSS_OK;
 }
 

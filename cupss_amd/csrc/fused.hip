@@ -279,10 +279,21 @@ __device__ __forceinline__ void target_sum(const FusedInst &a, const FusedTarget
             const TermRefDev &DM = a.dmul[TR.idx];
             float dm[E];
             multiplier_col<E>(a.pool, DM.pres0, DM.npres, DM.mult_i, cb, dm);
+            const int b = a.dbase[TR.idx];
+            if (b < 0) {
 #pragma unroll
-            for (int e = 0; e < E; e++) X[e] = apply_mult(dm[e], DM.mult_i, old[e]);
+                for (int e = 0; e < E; e++) X[e] = old[e];
+            } else {
+                const float2 *__restrict__ src = a.cs[b][parity];
+#pragma unroll
+                for (int e = 0; e < E; e++) X[e] = active ? gload(src, col + t + e * T) : make_float2(0.f, 0.f);
+            }
+#pragma unroll
+            for (int e = 0; e < E; e++) X[e] = apply_mult(dm[e], DM.mult_i, X[e]);
         } else {
-            const float2 *__restrict__ src = a.cs[TR.idx][parity];
+            // SRC_DYN_NEW: a stored dynamic field's value after the step, which
+            // this lane wrote to the other buffer earlier in the same pass
+            const float2 *__restrict__ src = a.cs[TR.idx][TR.src == SRC_DYN_NEW ? parity ^ 1 : parity];
 #pragma unroll
             for (int e = 0; e < E; e++) X[e] = active ? gload(src, col + t + e * T) : make_float2(0.f, 0.f);
         }
@@ -308,6 +319,74 @@ __device__ __forceinline__ void z_out(const FusedInst &a, int f, const Geom &g, 
         for (int e = 0; e < E; e++) gstore(a.Z[f], col + t + e * T, v[e]);
     }
     __syncthreads();
+}
+
+// sum over the pool prefactors [p0, p0 + n) of c q^(2 q2n) |q|^-invq per bin
+// (the implicit operator of a stored field)
+template <int E>
+__device__ __forceinline__ void pool_sum_col(const PresDev *pool, int p0, int n, const ColBins<E> &cb,
+                                             float (&sum)[E])
+{
+#pragma unroll
+    for (int e = 0; e < E; e++) sum[e] = 0.0f;
+    for (int i = 0; i < n; i++) {
+        const PresDev P = pool[p0 + i];
+        float th[E];
+#pragma unroll
+        for (int e = 0; e < E; e++) th[e] = P.c;
+        for (int k = 0; k < P.q2n; k++)
+#pragma unroll
+            for (int e = 0; e < E; e++) th[e] *= cb.q2[e];
+        if (P.invq != 0) {
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const float iq = col_iq(cb, e);
+                for (int k = 0; k < P.invq; k++) th[e] *= iq;
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < E; e++) sum[e] += th[e];
+    }
+}
+
+// 1/D(q), D = 1 - dt sum of the pool prefactors [p0, p0 + n) (the implicit
+// operator of a stored dynamic field, summed in implicit_col's order)
+template <int E>
+__device__ __forceinline__ void pool_implicit_col(const PresDev *pool, int p0, int n, float dt, const ColBins<E> &cb,
+                                                  float (&rD)[E])
+{
+    float D[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) D[e] = 1.0f;
+    for (int i = 0; i < n; i++) {
+        float th[E];
+        pool_sum_col<E>(pool, p0 + i, 1, cb, th);
+#pragma unroll
+        for (int e = 0; e < E; e++) D[e] -= dt * th[e];
+    }
+#pragma unroll
+    for (int e = 0; e < E; e++) rD[e] = __builtin_amdgcn_rcpf(D[e]);
+}
+
+// Z_f = IFFT_y(mask_f * L_f u) of a real field that is a dynamic field u or
+// derived from it
+template <int N, int E, int T>
+__device__ __forceinline__ void lz_out(const FusedInst &a, int f, const Geom &g, const ColBins<E> &cb,
+                                       const float2 (&u)[E], bool zero, int line, bool active, long col, float2 *L,
+                                       int t, const float2 *tws)
+{
+    float2 v[E];
+    const TermRefDev &LM = a.lmul[f];
+    if (LM.npres > 0) {
+        float lm[E];
+        multiplier_col<E>(a.pool, LM.pres0, LM.npres, LM.mult_i, cb, lm);
+#pragma unroll
+        for (int e = 0; e < E; e++) v[e] = apply_mult(lm[e], LM.mult_i, u[e]);
+    } else {
+#pragma unroll
+        for (int e = 0; e < E; e++) v[e] = u[e];
+    }
+    z_out<N, E, T>(a, f, g, cb, v, zero, line, active, col, L, t, tws);
 }
 
 // Hermitian projection of the columns kx = 0 and kx = sx/2 (2D): bin ky
@@ -348,7 +427,7 @@ struct FColCfg {
 
 // CONS: the instances may have stored constraint fields (compiled out
 // otherwise, which keeps the kernel within 128 VGPRs: 4 waves per SIMD)
-template <int N, int TWD, bool CONS>
+template <int N, int TWD, bool CONS, bool XDYN>
 __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const FusedInst *__restrict__ insts,
                                                                    const FusedBlock *__restrict__ blocks,
                                                                    int parity, uint32_t step_rel, int mode,
@@ -440,34 +519,49 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
 #pragma unroll
             for (int e = 0; e < E; e++) gstore(sout, col + t + e * T, nw[e]);
         }
-        // ---- stored constraints (setNotDynamic_k, src/field_kernels.cu:130-197):
-        // their values of step m+1, from the pre-step state of step m+1
+        // ---- stored fields: further dynamic fields (setDynamic_k) from their
+        // values before the step, then the constraints (setNotDynamic_k,
+        // src/field_kernels.cu:130-197): their values of step m+1, from the
+        // pre-step state of step m+1
         for (int ci = 0; ci < (CONS ? a.ncons : 0); ci++) {
             const FusedTarget &TG = a.tgt[1 + ci];
             float2 v[E];
             target_sum<N, E, T>(a, TG, cb, cur, nw, xv, false, parity, active, col, L, t, tws, v);
-            if (TG.nimp > 0) {
-                float lsum[E];
+            if (XDYN && TG.dyn) {
+                const float2 *__restrict__ uin = a.cs[ci][parity];
+                float2 ue[E];
 #pragma unroll
-                for (int e = 0; e < E; e++) lsum[e] = 0.0f;
-                for (int i = 0; i < TG.nimp; i++) {
-                    const PresDev P = a.pool[TG.imp0 + i];
-                    float th[E];
-#pragma unroll
-                    for (int e = 0; e < E; e++) th[e] = P.c;
-                    for (int k = 0; k < P.q2n; k++)
-#pragma unroll
-                        for (int e = 0; e < E; e++) th[e] *= cb.q2[e];
-                    if (P.invq != 0) {
-#pragma unroll
-                        for (int e = 0; e < E; e++) {
-                            const float iq = col_iq(cb, e);
-                            for (int k = 0; k < P.invq; k++) th[e] *= iq;
-                        }
-                    }
-#pragma unroll
-                    for (int e = 0; e < E; e++) lsum[e] += th[e];
+                for (int e = 0; e < E; e++) {
+                    ue[e] = make_float2(0.f, 0.f);
+                    const float2 u0 = active ? gload(uin, col + t + e * T) : make_float2(0.f, 0.f);
+                    v[e] = make_float2(u0.x + a.u.dt * v[e].x, u0.y + a.u.dt * v[e].y);
                 }
+                if (TG.noisy) {
+#pragma unroll
+                    for (int e = 0; e < E / 2; e++)
+                        spectral_noise_pair(g, kx, t + e * T, 0, (uint32_t)step, (uint32_t)(step >> 32), TG.stream,
+                                            TG.key0, TG.key1, ue[e], ue[e + E / 2]);
+#pragma unroll
+                    for (int e = 0; e < E; e++) {
+                        float pn = TG.noise_base;
+                        if (TG.noise_q2n != 0) pn *= sqrtf(ipow(cb.q2[e], TG.noise_q2n));
+                        if (TG.noise_invq != 0) pn *= sqrtf(ipow(col_iq(cb, e), TG.noise_invq));
+                        v[e].x += pn * ue[e].x;
+                        v[e].y += pn * ue[e].y;
+                    }
+                }
+                if (TG.nimp > 0) {
+                    float rD[E];
+                    pool_implicit_col<E>(a.pool, TG.imp0, TG.nimp, a.u.dt, cb, rD);
+#pragma unroll
+                    for (int e = 0; e < E; e++) {
+                        v[e].x *= rD[e];
+                        v[e].y *= rD[e];
+                    }
+                }
+            } else if (TG.nimp > 0) {
+                float lsum[E];
+                pool_sum_col<E>(a.pool, TG.imp0, TG.nimp, cb, lsum);
 #pragma unroll
                 for (int e = 0; e < E; e++)
                     if (!cb.zero[e]) {
@@ -483,6 +577,10 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
                 for (int e = 0; e < E; e++) gstore(cout, col + t + e * T, v[e]);
             }
             if (TG.z >= 0) z_out<N, E, T>(a, TG.z, g, cb, v, false, blk.line, active, col, L, t, tws);
+            if (XDYN && TG.dyn)
+                for (int f = 0; f < a.nreal; f++)
+                    if (a.zsrc[f] < 0 && a.lbase[f] == ci)
+                        lz_out<N, E, T>(a, f, g, cb, v, false, blk.line, active, col, L, t, tws);
         }
     } else {
         // prologue: the real fields of the current state
@@ -490,28 +588,22 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
         for (int e = 0; e < E; e++) nw[e] = cur[e];
         for (int ci = 0; ci < (CONS ? a.ncons : 0); ci++) {
             const int z = a.tgt[1 + ci].z;
-            if (z < 0) continue;
+            const bool dyn = XDYN && a.tgt[1 + ci].dyn != 0;
+            if (z < 0 && !dyn) continue;
             float2 v[E];
 #pragma unroll
             for (int e = 0; e < E; e++) v[e] = active ? gload(a.cs[ci][0], col + t + e * T) : make_float2(0.f, 0.f);
-            z_out<N, E, T>(a, z, g, cb, v, a.zero[z] != 0, blk.line, active, col, L, t, tws);
+            if (z >= 0) z_out<N, E, T>(a, z, g, cb, v, a.zero[z] != 0, blk.line, active, col, L, t, tws);
+            if (dyn)
+                for (int f = 0; f < a.nreal; f++)
+                    if (a.zsrc[f] < 0 && a.lbase[f] == ci)
+                        lz_out<N, E, T>(a, f, g, cb, v, a.zero[f] != 0, blk.line, active, col, L, t, tws);
         }
     }
     // real fields that are phi or derived from it: L_f phi_{m+1}
     for (int f = 0; f < a.nreal; f++) {
-        if (a.zsrc[f] >= 0) continue;
-        float2 v[E];
-        const TermRefDev &LM = a.lmul[f];
-        if (LM.npres > 0) {
-            float lm[E];
-            multiplier_col<E>(a.pool, LM.pres0, LM.npres, LM.mult_i, cb, lm);
-#pragma unroll
-            for (int e = 0; e < E; e++) v[e] = apply_mult(lm[e], LM.mult_i, nw[e]);
-        } else {
-#pragma unroll
-            for (int e = 0; e < E; e++) v[e] = nw[e];
-        }
-        z_out<N, E, T>(a, f, g, cb, v, !upd && a.zero[f] != 0, blk.line, active, col, L, t, tws);
+        if (a.zsrc[f] >= 0 || a.lbase[f] >= 0) continue;
+        lz_out<N, E, T>(a, f, g, cb, nw, !upd && a.zero[f] != 0, blk.line, active, col, L, t, tws);
     }
     if (slot) {
         __syncthreads();
@@ -590,9 +682,10 @@ __global__ __launch_bounds__(256) void k_ftrans_in(const FusedInst *__restrict__
         for (int c = 0; c < a.ncons; c++) tile_to_cm(s, a.g, x0, y0, a.cnat[c], a.cs[c][0]);
 }
 
-// column-major phi [fin] -> natural; stored constraints of the last step's
-// start ([fin^1]) -> natural; the derived fields of the last step's
-// starting phi ([fin^1]) in natural layout
+// column-major phi and stored dynamic fields [fin] -> natural; stored
+// constraints of the last step's start ([fin^1]) -> natural; the derived
+// fields of the last step's starting values of their base ([fin^1]) in
+// natural layout
 __global__ __launch_bounds__(256) void k_fsettle(const FusedInst *__restrict__ insts,
                                                  const FusedBlock *__restrict__ blocks, int fin)
 {
@@ -604,8 +697,10 @@ __global__ __launch_bounds__(256) void k_fsettle(const FusedInst *__restrict__ i
     tile_origin(a.g, blk.line, x0, y0);
     const TermRefDev copy{nullptr, 0, 0, 0, 0};
     tile_to_nat(s, a, x0, y0, a.spec[fin], a.nat, copy);
-    for (int c = 0; c < a.ncons; c++) tile_to_nat(s, a, x0, y0, a.cs[c][fin ^ 1], a.cnat[c], copy);
-    for (int d = 0; d < a.nder; d++) tile_to_nat(s, a, x0, y0, a.spec[fin ^ 1], a.dout[d], a.dmul[d]);
+    for (int c = 0; c < a.ncons; c++)
+        tile_to_nat(s, a, x0, y0, a.cs[c][a.tgt[1 + c].dyn ? fin : fin ^ 1], a.cnat[c], copy);
+    for (int d = 0; d < a.nder; d++)
+        tile_to_nat(s, a, x0, y0, a.dbase[d] < 0 ? a.spec[fin ^ 1] : a.cs[a.dbase[d]][fin ^ 1], a.dout[d], a.dmul[d]);
 }
 
 // ------------------------------------------------------------ elementwise
@@ -1059,6 +1154,13 @@ bool fused_supported(int sx, int sy, int sz)
 // wide row kernels (more than 3 real fields) for the sizes such systems use
 #define FUSED_WIDE_NC(NC) ((NC) >= 32 && (NC) <= 1024)
 
+bool fused_shapes_exist(int sx, int sy, int sz, int nreal)
+{
+    const int nc = sx / 2;
+    if (sz > 1) return nreal <= 3 || (nc >= 32 && nc <= 1024);
+    return !fused_row_shapes(nc, sy, nreal > 3).empty() && !fused_col_shapes(sy).empty();
+}
+
 template <int NC, int MAXR>
 static std::vector<int> row_shapes(int ny)
 {
@@ -1147,28 +1249,32 @@ hipError_t fused_rows_launch(int nc, int lpb, bool wide, const FusedInst *inst, 
 }
 
 template <int N>
-static hipError_t fcols_n(int twd, bool cons, const FusedInst *inst, const FusedBlock *blocks, int nblocks,
+static hipError_t fcols_n(int twd, int stored, const FusedInst *inst, const FusedBlock *blocks, int nblocks,
                           int parity, uint32_t step_rel, int mode, hipStream_t s, const float2 *sets,
                           unsigned long long *ts)
 {
-#define X(TWD)                                                                                                   \
-    if constexpr (FColCfg<N, TWD>::valid()) {                                                                   \
-        if (twd == TWD) {                                                                                        \
-            if (cons)                                                                                            \
-                hipLaunchKernelGGL((k_fcols<N, TWD, true>), dim3(nblocks), dim3(FColCfg<N, TWD>::THREADS), 0, s, \
-                                   inst, blocks, parity, step_rel, mode, sets, ts);                              \
-            else                                                                                                 \
-                hipLaunchKernelGGL((k_fcols<N, TWD, false>), dim3(nblocks), dim3(FColCfg<N, TWD>::THREADS), 0,   \
-                                   s, inst, blocks, parity, step_rel, mode, sets, ts);                           \
-            return hipGetLastError();                                                                            \
-        }                                                                                                        \
+#define L(CONS, XDYN, TWD)                                                                                    \
+    hipLaunchKernelGGL((k_fcols<N, TWD, CONS, XDYN>), dim3(nblocks), dim3(FColCfg<N, TWD>::THREADS), 0, s, inst, \
+                       blocks, parity, step_rel, mode, sets, ts)
+#define X(TWD)                                   \
+    if constexpr (FColCfg<N, TWD>::valid()) {   \
+        if (twd == TWD) {                        \
+            if (stored == 2)                     \
+                L(true, true, TWD);              \
+            else if (stored == 1)                \
+                L(true, false, TWD);             \
+            else                                 \
+                L(false, false, TWD);            \
+            return hipGetLastError();            \
+        }                                        \
     }
     FUSED_TWD(X)
 #undef X
+#undef L
     return hipErrorInvalidValue;
 }
 
-hipError_t fused_cols_launch(int ny, int twd, bool cons, const FusedInst *inst, const FusedBlock *blocks,
+hipError_t fused_cols_launch(int ny, int twd, int stored, const FusedInst *inst, const FusedBlock *blocks,
                              int nblocks, int parity, uint32_t step_rel, int mode, hipStream_t s,
                              unsigned long long *ts)
 {
@@ -1177,7 +1283,7 @@ hipError_t fused_cols_launch(int ny, int twd, bool cons, const FusedInst *inst, 
     switch (ny) {
 #define X(N) \
     case N:  \
-        return fcols_n<N>(twd, cons, inst, blocks, nblocks, parity, step_rel, mode, s, sets, ts);
+        return fcols_n<N>(twd, stored, inst, blocks, nblocks, parity, step_rel, mode, s, sets, ts);
         FUSED_COL_CASES(X)
 #undef X
     default:

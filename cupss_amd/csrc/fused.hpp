@@ -1,15 +1,16 @@
-// fused.hpp -- the fused two-kernel step for 2D systems with one dynamic
-// field phi (fused.hip), batched over any number of independent instances
-// (batch.hip), and the four-pass fused step of such 3D systems (one at a
-// time).
+// fused.hpp -- the fused two-kernel step for 2D systems (fused.hip),
+// batched over any number of independent instances (batch.hip), and the
+// four-pass fused step of 3D systems with one dynamic field (one at a time).
 //
-// The other fields of such a system are
-//   derived:  psi = M_psi phi (one single-field term of phi, no implicit
-//             operator, no noise) -- never stored during a run, evaluated
-//             from phi where needed;
-//   stored:   any other constraint field without noise whose product terms
-//             read only derived or stored fields -- kept as state, updated
-//             every pass (the "phase A/B" of the next step, see fused.hip).
+// The first dynamic field is phi; the fields of such a system are
+//   derived:  psi = M_psi u for a dynamic field u (one single-field term, no
+//             implicit operator, no noise) -- never stored during a run,
+//             evaluated from u where needed;
+//   stored:   the other dynamic fields (2D; noise drawn in spectral space
+//             only), then any other constraint field without noise whose
+//             product terms read only derived or stored constraint fields --
+//             kept as state, updated every pass (the "phase A/B" of the next
+//             step, see fused.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -46,7 +47,8 @@ enum FusedSrc : int {
     SRC_NEW = 1,      // phi_{m+1} (terms of constraint fields: phi's pre-step value of step m+1)
     SRC_CONST = 2,    // no factor: N at q = 0
     SRC_DERIVED = 3,  // derived field idx: M_idx phi_m
-    SRC_CONS = 4,     // stored constraint idx at step m
+    SRC_CONS = 4,     // stored field idx at step m (a dynamic one: its value before the step)
+    SRC_DYN_NEW = 5,  // stored dynamic field idx after the step (terms of constraint fields)
 };
 
 struct FusedTerm {
@@ -54,13 +56,19 @@ struct FusedTerm {
     int mult_i, pres0, npres;  // multiplier in the pool
 };
 
-// One field a pass updates: phi (target 0) or stored constraint c (1 + c)
+// One field a pass updates: phi (target 0) or stored field c (1 + c)
 struct FusedTarget {
     int nterms;
     FusedTerm term[FUSED_MAX_TTERMS];
     int g0, g1;      // its product groups [g0, g1)
-    int nimp, imp0;  // constraint: implicit prefactors in the pool (division for q != 0)
-    int z;           // constraint: index of its real field, -1 if none
+    int nimp, imp0;  // stored field: implicit prefactors in the pool (constraint: division for q != 0;
+                     // dynamic: D(q) = 1 - dt sum)
+    int z;           // stored field: index of its real field, -1 if none
+    int dyn;         // stored dynamic field: semi-implicit Euler step of its own value
+    int noisy;       // stored dynamic field: spectral noise, amplitude as UpdateArgs
+    float noise_base;
+    int noise_q2n, noise_invq;
+    uint32_t key0, key1, stream;
 };
 
 // Device-side descriptor of one instance (one system) of a fused batch.
@@ -101,12 +109,14 @@ struct FusedInst {
     int kmax[FUSED_MAX_REAL];                 // last column kx the dealias mask of field f keeps
     int kymax[FUSED_MAX_REAL];                // largest |ky| the mask keeps
     int zero[FUSED_MAX_REAL];                 // real field is zero in the prologue (step-0 state)
-    int zsrc[FUSED_MAX_REAL];                 // -1: L_f phi (lmul); c >= 0: stored constraint c
+    int zsrc[FUSED_MAX_REAL];                 // -1: L_f u (lmul, u = lbase); c >= 0: stored field c
+    int lbase[FUSED_MAX_REAL];                // zsrc -1: u is phi (-1) or stored dynamic field c
     TermRefDev lmul[FUSED_MAX_REAL];          // L_f (npres = 0: identity; pool)
-    // derived fields psi_d = P(M_d phi_old), written after a run
+    // derived fields psi_d = P(M_d u_old), written after a run
     int nder;
     float2 *dout[FUSED_MAX_DERIVED];
     TermRefDev dmul[FUSED_MAX_DERIVED];       // pool
+    int dbase[FUSED_MAX_DERIVED];             // u: phi (-1) or stored dynamic field c
 };
 
 // Work item of one workgroup: instance and first row / column / tile / bin
@@ -117,6 +127,9 @@ struct FusedBlock {
 };
 
 bool fused_supported(int sx, int sy, int sz);
+// the row and column (2D) or row (3D) kernels exist for this grid with
+// nreal real fields (more than 3: the wide row kernel)
+bool fused_shapes_exist(int sx, int sy, int sz, int nreal);
 
 // Tile shapes a size class can be launched with: rows per workgroup of the
 // row kernel (nc = sx/2; wide = more than 3 real fields) and columns per
@@ -131,8 +144,9 @@ hipError_t fused_rows_launch(int nc, int lpb, bool wide, const FusedInst *inst, 
 // (ts: null, or FUSED_TS_SLOTS pairs {earliest start, latest end} of
 // wall-clock ticks, FUSED_TS_STRIDE apart, that the launch's workgroups
 // min/max-update; the launch spans min(starts) .. max(ends))
-// (cons: some instance of the launch has stored constraint fields)
-hipError_t fused_cols_launch(int ny, int twd, bool cons, const FusedInst *inst, const FusedBlock *blocks, int nblocks,
+// (stored: 0 = no instance of the launch has stored fields, 1 = some have
+// stored constraint fields, 2 = some have stored dynamic fields)
+hipError_t fused_cols_launch(int ny, int twd, int stored, const FusedInst *inst, const FusedBlock *blocks, int nblocks,
                              int parity, uint32_t step_rel, int mode, hipStream_t s, unsigned long long *ts = nullptr);
 // the slots of n timed launches to {~0, 0} (a kernel, so that graphs can hold it)
 hipError_t fused_ts_reset_launch(unsigned long long *ts, int n, hipStream_t s);
@@ -153,8 +167,8 @@ hipError_t fused3d_derived_launch(const FusedInst *inst, long nh, int parity, hi
 // natural spectra -> column-major buffer 0 (phi and the stored constraints;
 // blocks: one tile each)
 hipError_t fused_transpose_in_launch(const FusedInst *inst, const FusedBlock *blocks, int nblocks, hipStream_t s);
-// column-major phi [fin] and stored constraints [fin^1] -> natural, derived
-// fields from phi [fin^1]
+// column-major phi and stored dynamic fields [fin], stored constraints
+// [fin^1] -> natural; derived fields from their base's [fin^1]
 hipError_t fused_settle_launch(const FusedInst *inst, const FusedBlock *blocks, int nblocks, int fin,
                                hipStream_t s);
 // nsteps steps of instances with no products, no stored constraints and no

@@ -729,16 +729,34 @@ static int general_advance(cupss_system s, int nsteps)
 }
 
 
+// The fused steps draw real-space noise for phi of a 2D system only: the 3D
+// step and the other dynamic fields draw theirs in spectral space.
+bool cupss::fused_noise_ok(cupss_system s)
+{
+    if (!s->plan.ok || s->noise_mode == 1) return true;
+    if (s->g.sz > 1 && s->f[s->plan.phi].noisy) return false;
+    for (int k = 0; k < s->plan.nxdyn; k++)
+        if (s->f[s->plan.cons[k]].noisy) return false;
+    return true;
+}
+
+// a dynamic field of the fused plan holds a stale dealiased copy
+bool cupss::fused_stale(cupss_system s)
+{
+    if (s->f[s->plan.phi].dstate == 3) return true;
+    for (int k = 0; k < s->plan.nxdyn; k++)
+        if (s->f[s->plan.cons[k]].dstate == 3) return true;
+    return false;
+}
+
 extern "C" int cupss_system_advance(cupss_system s, int nsteps)
 {
     if (int r = check_sys(s)) return r;
     if (!s->prepared) return fail(CUPSS_ERR_STATE, "cupss_system_prepare must be called before advancing");
     if (nsteps <= 0) return CUPSS_OK;
-    // the 3D fused step draws noise in spectral space only
-    const bool real_noise_3d = s->g.sz > 1 && s->plan.ok && s->f[s->plan.phi].noisy && s->noise_mode == 0;
-    if (s->path_mode == 0 && s->plan.ok && !hooks_present(s) && !real_noise_3d) {
+    if (s->path_mode == 0 && s->plan.ok && !hooks_present(s) && fused_noise_ok(s)) {
         // a stale dealiased copy is consumed by one general step first
-        if (s->f[s->plan.phi].dstate == 3) {
+        if (fused_stale(s)) {
             if (int r = general_steps(s, 1)) return r;
             nsteps -= 1;
         }
@@ -810,7 +828,7 @@ extern "C" int cupss_system_set_noise_mode(cupss_system s, int mode)
 extern "C" int cupss_system_fused_eligible(cupss_system s)
 {
     if (check_sys(s)) return 0;
-    return s->plan.ok && s->g.sz == 1 && !hooks_present(s) ? 1 : 0;
+    return s->plan.ok && s->g.sz == 1 && !hooks_present(s) && fused_noise_ok(s) ? 1 : 0;
 }
 
 // ------------------------------------------------------------ fused path
@@ -846,30 +864,43 @@ static void plan_fused(cupss_system s)
     P = FusedPlan();
     if (!fused_supported(s->g.sx, s->g.sy, s->g.sz)) return;
     const int nf = (int)s->f.size();
+    // the first dynamic field is phi; in 2D the others are stored fields
+    // (their noise drawn in spectral space only: fused_noise_ok)
     int phi = -1;
+    std::vector<char> dyn(nf, 0);
     for (int i = 0; i < nf; i++)
         if (s->f[i].dynamic) {
-            if (phi >= 0) return;
-            phi = i;
+            dyn[i] = 1;
+            if (phi < 0) {
+                phi = i;
+            } else if (s->g.sz > 1) {
+                return;
+            }
         }
     if (phi < 0) return;
     std::vector<char> derived(nf, 0), stored(nf, 0);
+    std::vector<int> base(nf, -1);
     for (int i = 0; i < nf; i++) {
         if (i == phi) continue;
+        if (dyn[i]) {
+            stored[i] = 1;
+            continue;
+        }
         const FieldRec &F = s->f[i];
         if (F.noisy) return;
         if (F.imp.empty() && F.terms.size() == 1) {
             const TermRec &T = s->t[F.terms[0]];
-            if (T.fac.size() == 1 && T.fac[0] == phi && (int)T.pres.size() <= FUSED_MAX_POOL / 8) {
+            if (T.fac.size() == 1 && dyn[T.fac[0]] && (int)T.pres.size() <= FUSED_MAX_POOL / 8) {
                 derived[i] = 1;
+                base[i] = T.fac[0];
                 continue;
             }
         }
         if (F.terms.empty()) return;  // keeps its value: the general program's case
-        stored[i] = 1;
+        stored[i] = 2;
     }
-    // product factors: phi's products read anything; stored fields' products
-    // read derived or stored fields only
+    // product factors: dynamic fields' products read anything; stored
+    // constraints' products read derived or stored constraint fields only
     std::vector<std::vector<int>> prods_of(nf);
     for (int i = 0; i < nf; i++) {
         if (derived[i]) continue;
@@ -879,7 +910,7 @@ static void plan_fused(cupss_system s)
             if (T.fac.size() >= 2) {
                 if ((int)T.fac.size() > 8) return;
                 for (int c : T.fac) {
-                    if (i != phi && c == phi) return;
+                    if (!dyn[i] && dyn[c]) return;
                     if (std::find(P.real.begin(), P.real.end(), c) == P.real.end()) P.real.push_back(c);
                 }
                 prods_of[i].push_back(ti);
@@ -890,8 +921,12 @@ static void plan_fused(cupss_system s)
         if (single > FUSED_MAX_TTERMS) return;
     }
     if ((int)P.real.size() > FUSED_MAX_REAL) return;
+    if (!fused_shapes_exist(s->g.sx, s->g.sy, s->g.sz, (int)P.real.size())) return;
+    for (int kind = 1; kind <= 2; kind++)
+        for (int i = 0; i < nf; i++)
+            if (stored[i] == kind) P.cons.push_back(i);
     for (int i = 0; i < nf; i++)
-        if (stored[i]) P.cons.push_back(i);
+        if (stored[i] == 1) P.nxdyn++;
     if ((int)P.cons.size() > FUSED_MAX_CONS) return;
     if (s->g.sz > 1 && !P.cons.empty()) return;  // the 3D step has no stored fields
     // groups: phi's first, then each stored field's
@@ -945,7 +980,10 @@ static void plan_fused(cupss_system s)
     }
     if ((int)P.groups.size() > FUSED_MAX_GROUPS || nprod > FUSED_MAX_PRODS) return;
     for (int i = 0; i < nf; i++)
-        if (derived[i]) P.derived.push_back(i);
+        if (derived[i]) {
+            P.derived.push_back(i);
+            P.dbase.push_back(base[i]);
+        }
     if ((int)P.derived.size() > FUSED_MAX_DERIVED) return;
     // the prefactor pool must hold every multiplier of the instance
     size_t pool = 0;
@@ -963,8 +1001,8 @@ static void plan_fused(cupss_system s)
 
 static bool same_plan_shape(const FusedPlan &a, const FusedPlan &b)
 {
-    if (a.ok != b.ok || a.phi != b.phi || a.real != b.real || a.derived != b.derived || a.cons != b.cons ||
-        a.groups.size() != b.groups.size())
+    if (a.ok != b.ok || a.phi != b.phi || a.real != b.real || a.derived != b.derived || a.dbase != b.dbase ||
+        a.cons != b.cons || a.nxdyn != b.nxdyn || a.groups.size() != b.groups.size())
         return false;
     for (size_t i = 0; i < a.groups.size(); i++) {
         const FusedGroup &x = a.groups[i], &y = b.groups[i];

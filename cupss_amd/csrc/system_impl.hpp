@@ -67,8 +67,10 @@ struct FusedPlan {
     int phi = -1;
     std::vector<int> real;            // fields read in real space by the products
     std::vector<FusedGroup> groups;   // product groups, phi's first, then by stored field
-    std::vector<int> derived;         // constraint fields psi = M_psi phi
-    std::vector<int> cons;            // stored constraint fields
+    std::vector<int> derived;         // constraint fields psi = M_psi u, u dynamic
+    std::vector<int> dbase;           // u of each derived field
+    std::vector<int> cons;            // stored fields: the other dynamic fields (nxdyn), then constraints
+    int nxdyn = 0;
 };
 
 }  // namespace cupss
@@ -129,6 +131,10 @@ int materialize(cupss_system s, int fi);
 int general_steps(cupss_system s, int nsteps);
 // phase A/B of the reference step (constraint fields) with the general program
 int general_constraint_phase(cupss_system s);
+// the fused plan's noise can be drawn as the noise mode asks
+bool fused_noise_ok(cupss_system s);
+// a dynamic field of the fused plan holds a stale dealiased copy
+bool fused_stale(cupss_system s);
 // fused path (batch.hip)
 int fused_steps(cupss_system s, int nsteps);
 void fused_release(cupss_system s);

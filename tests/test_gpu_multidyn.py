@@ -1,0 +1,173 @@
+"""GPU: systems with several dynamic fields on the fused 2D step and in the
+batched step.  The first dynamic field is phi; the others are stored fields
+that the column pass advances with their own semi-implicit Euler step
+(setDynamic_k, src/field_kernels.cu:199-227), in the reference's phase
+order (src/evolver.cpp:199-226): constraints first, from the dynamic fields
+after the previous step, then every dynamic field from the same pre-step
+state.
+
+* Gray-Scott reaction-diffusion (u, v: products u v^2 in both, a constant
+  and linear terms) and a coupled system that reads every source kind a
+  second dynamic field adds -- a derived field of it (gx = iqx psi), a
+  constraint solved from both dynamic fields (w (1 + q^2) = psi + 0.3 phi),
+  products of a dynamic field with that constraint -- against the oracle's
+  FP32 scheme (relative L2 <= 1e-4 after 100 steps, as the one-field
+  cases).
+* Runs split into pieces (resume of the column-major state and of the
+  stored fields) are bit-identical to one run, and a batch member is
+  bit-identical to the same system advanced alone.
+* Noise on a second dynamic field: drawn in spectral space, the fused step
+  equals the general program (same Philox counters; FP32 rounding only);
+  in real space (which the fused row pass draws for phi only) the system
+  takes the general program, bit-identical to forcing it.
+* A grid whose row kernel does not exist for the number of real fields
+  (more than 3 at sx = 32) takes the general program and still matches the
+  oracle.
+"""
+import numpy as np
+import pytest
+
+import cupss_amd as C
+from oracle import oracle as O
+from tests.helpers import build_system, rel_l2, spec_cahn_hilliard
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu(spec, path=0, noise_mode=0, profile=False):
+    g, ids = build_system(C.System, C.pres, spec)
+    g.set_path(path)
+    g.set_noise_mode(noise_mode)
+    if profile:
+        g.profile_kernel(True)
+    return g, ids
+
+
+def spec_gray_scott(n=64, seed=31, F=0.035, k=0.065):
+    """dt u + 0.16 q^2 u = -u v^2 + F (1 - u);  dt v + 0.08 q^2 v = u v^2 - (F + k) v"""
+    rng = np.random.default_rng(seed)
+    u = np.ones((n, n), np.float32)
+    v = np.zeros((n, n), np.float32)
+    c = slice(n // 2 - n // 8, n // 2 + n // 8)
+    u[c, c] = 0.5
+    v[c, c] = 0.25
+    u += rng.uniform(-0.02, 0.02, (n, n)).astype(np.float32)
+    v += rng.uniform(-0.02, 0.02, (n, n)).astype(np.float32)
+    terms = [("u", ["u", "v", "v"], [(-1.0,)]), ("u", [], [(F,)]), ("u", ["u"], [(-F,)]),
+             ("v", ["u", "v", "v"], [(1.0,)]), ("v", ["v"], [(-(F + k),)])]
+    return {"dims": (n, n, 1), "dt": 0.5, "fields": [("u", True), ("v", True)],
+            "implicit": {"u": [(-0.16, 1)], "v": [(-0.08, 1)]}, "terms": terms,
+            "init": {"u": u.ravel(), "v": v.ravel()}}
+
+
+def spec_coupled(dims=(64, 32, 1), seed=32, noise=None):
+    """dt phi + q^2 phi = -0.2 q^2 phi^3 + 0.5 gx phi + 0.2 w
+       dt psi + 0.5 q^2 psi = 0.3 iqx phi - 0.1 psi w
+       gx = iqx psi;   w (1 + q^2) = psi + 0.3 phi"""
+    n = dims[0] * dims[1] * dims[2]
+    rng = np.random.default_rng(seed)
+    terms = [("phi", ["phi", "phi", "phi"], [(-0.2, 1)]), ("phi", ["gx", "phi"], [(0.5,)]), ("phi", ["w"], [(0.2,)]),
+             ("psi", ["phi"], [(0.3, 0, 1)]), ("psi", ["psi", "w"], [(-0.1,)]),
+             ("gx", ["psi"], [(1.0, 0, 1)]),
+             ("w", ["psi"], [(1.0,)]), ("w", ["phi"], [(0.3,)])]
+    spec = {"dims": dims, "dt": 0.01,
+            "fields": [("phi", True), ("psi", True), ("gx", False), ("w", False)],
+            "implicit": {"phi": [(-1.0, 1)], "psi": [(-0.5, 1)], "w": [(1.0,), (1.0, 1)]},
+            "terms": terms,
+            "init": {"phi": rng.uniform(-0.2, 0.2, n).astype(np.float32),
+                     "psi": rng.uniform(-0.2, 0.2, n).astype(np.float32)}}
+    if noise is not None:
+        spec["noise"] = {"psi": ((noise,), seed + 1)}
+    return spec
+
+
+CASES = {"gray_scott": (spec_gray_scott, ("u", "v")),
+         "coupled": (spec_coupled, ("phi", "psi", "gx", "w"))}
+
+
+def assert_fused_stored_dynamic(g):
+    name = g.kernel_time()[2]
+    assert name.startswith("k_fcols<") and name.endswith("true, true>"), name
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_multi_dynamic_fused_vs_oracle(case):
+    mk, names = CASES[case]
+    spec = mk()
+    print("several dynamic fields, fused vs oracle:", case, flush=True)
+    g, gid = gpu(spec, profile=True)
+    o, oid = build_system(O.System, O.pres, dict(spec, precision="f32"))
+    g.step(100)
+    o.step(100)
+    assert_fused_stored_dynamic(g)
+    for f in names:
+        got, ref = g.get_real(gid[f]), o.get_real(oid[f])
+        assert np.all(np.isfinite(got))
+        err = rel_l2(got, ref)
+        print("  %-3s gpu-vs-f32 %.3e" % (f, err), flush=True)
+        assert err <= 1e-4, f
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_multi_dynamic_split_runs_and_batch_bitwise(case):
+    mk, names = CASES[case]
+    spec = mk()
+    one, ia = gpu(spec)
+    one.step(30)
+    split, ib = gpu(spec)
+    for n in (7, 1, 12, 10):
+        split.step(n)
+    for f in names:
+        assert np.array_equal(one.get_real(ia[f]), split.get_real(ib[f])), f
+    # a batch of this system, a one-field system and the other case
+    other = spec_coupled((64, 64, 1), seed=40) if case == "gray_scott" else spec_gray_scott(32, seed=41)
+    others = [spec_cahn_hilliard((32, 64, 1), seed=42), other]
+    batch = C.Batch(graph_steps=4)
+    members = []
+    for sp in [spec] + others:
+        g, ids = gpu(sp)
+        batch.add(g)
+        members.append((g, ids, sp))
+    batch.step(30)
+    for g, ids, sp in members:
+        alone, ia2 = gpu(sp)
+        alone.step(30)
+        for f in ids:
+            assert np.array_equal(alone.get_real(ia2[f]), g.get_real(ids[f])), f
+
+
+def test_noise_on_second_dynamic_field():
+    spec = spec_coupled(noise=0.02)
+    # spectral noise: the fused step, against the general program
+    runs = {}
+    for path in (0, 1):
+        g, ids = gpu(spec, path=path, noise_mode=1, profile=path == 0)
+        g.step(40)
+        if path == 0:
+            assert_fused_stored_dynamic(g)
+        runs[path] = (g, ids)
+    for f in ("phi", "psi", "w"):
+        e = rel_l2(runs[0][0].get_real(runs[0][1][f]), runs[1][0].get_real(runs[1][1][f]))
+        print("  spectral noise, %-3s fused-vs-general %.3e" % (f, e), flush=True)
+        assert e <= 1e-5, f
+    # real-space noise: the general program whatever the path setting
+    out = []
+    for path in (0, 1):
+        g, ids = gpu(spec, path=path, noise_mode=0)
+        assert not g.fused_eligible
+        g.step(20)
+        out.append(g.get_real(ids["psi"]))
+    assert np.array_equal(out[0], out[1])
+
+
+def test_no_row_kernel_for_the_grid_takes_the_general_program():
+    spec = spec_coupled((32, 64, 1), seed=44)  # 4 real fields, sx/2 = 16: no wide row kernel
+    g, gid = gpu(spec)
+    assert not g.fused_eligible
+    o, oid = build_system(O.System, O.pres, dict(spec, precision="f32"))
+    g.step(50)
+    o.step(50)
+    for f in ("phi", "psi"):
+        err = rel_l2(g.get_real(gid[f]), o.get_real(oid[f]))
+        print("  32x64, %-3s general-vs-f32 %.3e" % (f, err), flush=True)
+        assert err <= 1e-4, f
