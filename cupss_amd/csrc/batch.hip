@@ -260,8 +260,7 @@ static int describe(cupss_system s, FusedInst &a, bool natural = false)
     int np = 0;
     auto index_of = [](const std::vector<int> &v, int x) { return (int)(std::find(v.begin(), v.end(), x) - v.begin()); };
     // derived fields: one multiplier each (also L_f of their real fields)
-    if (!fused_noise_ok(s))
-        return fail(CUPSS_ERR_UNSUPPORTED, "the fused step draws the noise of a second dynamic field in spectral space only");
+    if (!fused_noise_ok(s)) return fail(CUPSS_ERR_UNSUPPORTED, "the fused 3D step draws noise in spectral space only");
     a.nder = (int)P.derived.size();
     for (size_t d = 0; d < P.derived.size(); d++) {
         if (int r = pool_term(a, np, s->t[s->f[P.derived[d]].terms[0]].pres, a.dmul[d])) return r;
@@ -323,13 +322,14 @@ static int describe(cupss_system s, FusedInst &a, bool natural = false)
         TG.dyn = 0;
         TG.noisy = 0;
         if (k > 0 && (int)k - 1 < P.nxdyn) {
-            // a further dynamic field: its noise (spectral) and D(q) from the
-            // implicit prefactors below
+            // a further dynamic field: its noise and D(q) from the implicit
+            // prefactors below
             UpdateArgs U;
             build_update(s, f, U);
             const FieldRec &G = s->f[f];
             TG.dyn = 1;
-            TG.noisy = G.noisy ? 1 : 0;
+            TG.noisy = !G.noisy ? NOISE_NONE : (s->noise_mode == 1 ? NOISE_SPECTRAL : NOISE_REAL);
+            TG.eta = G.eta;
             TG.noise_base = U.noise_base;
             TG.noise_q2n = U.noise_q2n;
             TG.noise_invq = U.noise_invq;
@@ -436,7 +436,9 @@ static int upload_blocks(const std::vector<FusedBlock> &items, FusedBlock **dst,
 // bytes of the pipeline buffers one transform instance touches per step
 static double inst_bytes(const FusedInst &a)
 {
-    const int arrays = 2 + 2 * a.ncons + a.nreal + a.ngroups + (a.noise_mode == NOISE_REAL ? 1 : 0);
+    int arrays = 2 + 2 * a.ncons + a.nreal + a.ngroups + (a.noise_mode == NOISE_REAL ? 1 : 0);
+    for (int c = 0; c < a.ncons; c++)
+        if (a.tgt[1 + c].dyn && a.tgt[1 + c].noisy == NOISE_REAL) arrays++;
     return 8.0 * (double)a.g.nh * arrays;
 }
 
@@ -1164,8 +1166,6 @@ extern "C" int cupss_batch_add(cupss_batch b, cupss_system s)
     for (auto &F : s->f)
         if (F.real_hook || F.spec_hook || F.real_dev || F.spec_dev)
             return fail(CUPSS_ERR_UNSUPPORTED, "systems with callbacks cannot be batched");
-    if (!fused_noise_ok(s))
-        return fail(CUPSS_ERR_UNSUPPORTED, "the batched step draws the noise of a second dynamic field in spectral space only");
     if (fused_stale(s)) {
         // consume a stale dealiased copy with one general step, as a lone system would
         if (int r = general_steps(s, 1)) return r;

@@ -213,7 +213,25 @@ __global__ __launch_bounds__((FRowCfg<NC, LPB, MAXR>::THREADS)) void k_frows(con
         tile_store<NC, LPB, RS, TH>(a.X[gi], tile, sy, y0);
         __syncthreads();
     }
-    if (a.noise_mode == NOISE_REAL) {
+    // real-space noise of phi (source -1) and of the stored dynamic fields
+    // that draw it (their own key, stream and buffer)
+    for (int src = -1; src < a.ncons; src++) {
+        uint32_t stream, key0, key1;
+        float2 *eta;
+        if (src < 0) {
+            if (a.noise_mode != NOISE_REAL) continue;
+            stream = a.stream;
+            key0 = a.key0;
+            key1 = a.key1;
+            eta = a.eta;
+        } else {
+            const FusedTarget &TG = a.tgt[1 + src];
+            if (!TG.dyn || TG.noisy != NOISE_REAL) continue;
+            stream = TG.stream;
+            key0 = TG.key0;
+            key1 = TG.key1;
+            eta = TG.eta;
+        }
         // Cells 2k, 2k+1 of row y0+l, k = t + e*T.  The Philox call with
         // counter (row*NC + k) >> 1 serves k and k^1, which sits in lane
         // t^1 (T is even): the even lane computes the calls of elements
@@ -226,8 +244,8 @@ __global__ __launch_bounds__((FRowCfg<NC, LPB, MAXR>::THREADS)) void k_frows(con
 #pragma unroll
         for (int e2 = 0; e2 < E / 2; e2++) {
             const uint32_t k = (uint32_t)((t & ~1) + (even ? e2 : e2 + E / 2) * T);
-            const Philox4 o = philox4x32_10((rowbase + k) >> 1, (uint32_t)step, (uint32_t)(step >> 32), a.stream,
-                                            a.key0, a.key1);
+            const Philox4 o = philox4x32_10((rowbase + k) >> 1, (uint32_t)step, (uint32_t)(step >> 32), stream,
+                                            key0, key1);
             const float2 ze = box_muller(o.x[0], o.x[1]), zo = box_muller(o.x[2], o.x[3]);
             const float2 own = even ? ze : zo, snd = even ? zo : ze;
             const float2 rcv = make_float2(__shfl_xor(snd.x, 1), __shfl_xor(snd.y, 1));
@@ -235,7 +253,8 @@ __global__ __launch_bounds__((FRowCfg<NC, LPB, MAXR>::THREADS)) void k_frows(con
             v[e2 + E / 2] = even ? rcv : own;
         }
         r2c_tile<NC, E, T>(v, L, t, tws);
-        tile_store<NC, LPB, RS, TH>(a.eta, tile, sy, y0);
+        tile_store<NC, LPB, RS, TH>(eta, tile, sy, y0);
+        __syncthreads();  // the tile is the next source's transform LDS
     }
 }
 
@@ -536,11 +555,18 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
                     const float2 u0 = active ? gload(uin, col + t + e * T) : make_float2(0.f, 0.f);
                     v[e] = make_float2(u0.x + a.u.dt * v[e].x, u0.y + a.u.dt * v[e].y);
                 }
-                if (TG.noisy) {
+                if (TG.noisy == NOISE_REAL) {
+#pragma unroll
+                    for (int e = 0; e < E; e++) ue[e] = active ? gload(TG.eta, col + t + e * T) : make_float2(0.f, 0.f);
+                    fft_line<N, E, -1, N>(ue, L, t, tws);
+                    __syncthreads();
+                } else if (TG.noisy == NOISE_SPECTRAL) {
 #pragma unroll
                     for (int e = 0; e < E / 2; e++)
                         spectral_noise_pair(g, kx, t + e * T, 0, (uint32_t)step, (uint32_t)(step >> 32), TG.stream,
                                             TG.key0, TG.key1, ue[e], ue[e + E / 2]);
+                }
+                if (TG.noisy) {
 #pragma unroll
                     for (int e = 0; e < E; e++) {
                         float pn = TG.noise_base;

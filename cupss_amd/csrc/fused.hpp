@@ -6,8 +6,8 @@
 //   derived:  psi = M_psi u for a dynamic field u (one single-field term, no
 //             implicit operator, no noise) -- never stored during a run,
 //             evaluated from u where needed;
-//   stored:   the other dynamic fields (2D; noise drawn in spectral space
-//             only), then any other constraint field without noise whose
+//   stored:   the other dynamic fields (2D), then any other constraint
+//             field without noise whose
 //             product terms read only derived or stored constraint fields --
 //             kept as state, updated every pass (the "phase A/B" of the next
 //             step, see fused.hip).
@@ -65,10 +65,11 @@ struct FusedTarget {
                      // dynamic: D(q) = 1 - dt sum)
     int z;           // stored field: index of its real field, -1 if none
     int dyn;         // stored dynamic field: semi-implicit Euler step of its own value
-    int noisy;       // stored dynamic field: spectral noise, amplitude as UpdateArgs
+    int noisy;       // stored dynamic field: its noise mode (NoiseMode), amplitude as UpdateArgs
     float noise_base;
     int noise_q2n, noise_invq;
     uint32_t key0, key1, stream;
+    float2 *eta;     // NOISE_REAL: R2C_x(xi) rows, column-major (row pass)
 };
 
 // Device-side descriptor of one instance (one system) of a fused batch.

@@ -729,15 +729,11 @@ static int general_advance(cupss_system s, int nsteps)
 }
 
 
-// The fused steps draw real-space noise for phi of a 2D system only: the 3D
-// step and the other dynamic fields draw theirs in spectral space.
+// The fused 3D step draws noise in spectral space only.
 bool cupss::fused_noise_ok(cupss_system s)
 {
     if (!s->plan.ok || s->noise_mode == 1) return true;
-    if (s->g.sz > 1 && s->f[s->plan.phi].noisy) return false;
-    for (int k = 0; k < s->plan.nxdyn; k++)
-        if (s->f[s->plan.cons[k]].noisy) return false;
-    return true;
+    return !(s->g.sz > 1 && s->f[s->plan.phi].noisy);
 }
 
 // a dynamic field of the fused plan holds a stale dealiased copy
@@ -865,7 +861,6 @@ static void plan_fused(cupss_system s)
     if (!fused_supported(s->g.sx, s->g.sy, s->g.sz)) return;
     const int nf = (int)s->f.size();
     // the first dynamic field is phi; in 2D the others are stored fields
-    // (their noise drawn in spectral space only: fused_noise_ok)
     int phi = -1;
     std::vector<char> dyn(nf, 0);
     for (int i = 0; i < nf; i++)
