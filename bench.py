@@ -104,6 +104,8 @@ WORKLOADS = {
     "ensemble": dict(name="batched EW/KPZ ensemble, 512 instances of 64^2..512^2", cpu_steps=0),
     "serve": dict(name="served model-H stream, Poisson arrivals, sides 128..1024", cpu_steps=0),
     "modelh": dict(build=None, dims=(1024, 1024, 1), name="2D model H 1024^2 (single instance)", cpu_steps=1),
+    "grayscott": dict(build=None, dims=(1024, 1024, 1), name="2D Gray-Scott 1024^2, two dynamic fields (single instance)",
+                      cpu_steps=15, dt=0.5, dynamic=2),
 }
 
 
@@ -158,14 +160,15 @@ PMC_TRAFFIC = {
 }
 
 
-def roofline(value, kernel, per, workload=None):
+def roofline(value, kernel, per, workload=None, dynamic=1):
     """The dominant kernel's roofline entry: the launch that performs the
     state update (one read and one write of each grid point's spectrum:
-    8 B per point) of the class with the most points, its average duration
-    from the wall-clock stamps its workgroups record inside the timed run."""
+    8 B per point and dynamic field) of the class with the most points, its
+    average duration from the wall-clock stamps its workgroups record inside
+    the timed run."""
     ms, pts, name = kernel
-    achieved = BYTES_PER_POINT_STEP * pts / (ms / 1e3) / 1e9
-    whole = BYTES_PER_POINT_STEP * value / 1e9
+    achieved = dynamic * BYTES_PER_POINT_STEP * pts / (ms / 1e3) / 1e9
+    whole = dynamic * BYTES_PER_POINT_STEP * value / 1e9
     traffic = None
     for (wl, prefix), b in PMC_TRAFFIC.items():
         if wl == workload and name.replace(" ", "").startswith(prefix):
@@ -419,10 +422,13 @@ def main():
     dims = wl["dims"]
     n = dims[0] * dims[1] * dims[2]
     x0 = np.random.default_rng(1).uniform(-0.1, 0.1, n).astype(np.float32)
-    if args.workload == "modelh":
-        from cupss_amd.systems import build, spec_model_h
-        spec = spec_model_h(dims[0], seed=1)
-        spec["init"] = {"phi": x0}
+    if args.workload in ("modelh", "grayscott"):
+        from cupss_amd.systems import build, spec_gray_scott, spec_model_h
+        if args.workload == "modelh":
+            spec = spec_model_h(dims[0], seed=1)
+            spec["init"] = {"phi": x0}
+        else:
+            spec = spec_gray_scott(dims[0], seed=1)
         wl["build"] = lambda S, P, dims, x0, **kw: build(dict(spec, **({"precision": kw["precision"]} if kw else {})),
                                                         S, P)
     s, phi = wl["build"](C.System, C.pres, dims, x0)
@@ -457,7 +463,8 @@ def main():
         path = "general step program"
     sec = ms / 1e3
     value = n * args.steps / sec
-    achieved = BYTES_PER_POINT_STEP * value / 1e9
+    ndyn = wl.get("dynamic", 1)
+    achieved = ndyn * BYTES_PER_POINT_STEP * value / 1e9
     out = {
         "metric": "grid-point-steps/s",
         "value": value,
@@ -471,14 +478,14 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": wl["name"], "grid": list(dims), "dt": 0.01, "path": path,
+        "config": {"workload": wl["name"], "grid": list(dims), "dt": wl.get("dt", 0.01), "path": path,
                    "ms_per_step_one_step_per_call": per_call_ms},
-        "roofline": (roofline(value, kernel, "8 B per grid point of the timed launch (the state update)",
-                              args.workload)
+        "roofline": (roofline(value, kernel, "%d B per grid point of the timed launch (the state update of %d "
+                              "dynamic field%s)" % (8 * ndyn, ndyn, "s" if ndyn > 1 else ""), args.workload, ndyn)
                      if kernel is not None else
                      {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                      "per": "whole step (all kernels of one advanceTime), 8 B/point algorithmic"}),
+                      "per": "whole step (all kernels of one advanceTime), %d B/point algorithmic" % (8 * ndyn)}),
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, x0)

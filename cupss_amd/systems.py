@@ -96,3 +96,23 @@ def spec_model_h(n, a=-1.0, b=1.0, k=4.0, eta=1.0, friction=0.0, ka=-4.0, dt=0.0
     imp = {"phi": [(-a, 1), (-k, 2)], "vx": [(friction,), (eta, 1)], "vy": [(friction,), (eta, 1)]}
     return {"dims": (n, n, 1), "dt": dt, "fields": fields, "implicit": imp, "terms": terms,
             "init": {"phi": x0}}
+
+
+def spec_gray_scott(n=64, seed=31, F=0.035, k=0.065, dt=0.5):
+    """Gray-Scott reaction-diffusion, two dynamic fields:
+    dt u + 0.16 q^2 u = -u v^2 + F (1 - u);  dt v + 0.08 q^2 v = u v^2 - (F + k) v,
+    from u = 1, v = 0 with a square of u = 1/2, v = 1/4 in the middle plus
+    U(-0.02, 0.02) on both."""
+    rng = np.random.default_rng(seed)
+    u = np.ones((n, n), np.float32)
+    v = np.zeros((n, n), np.float32)
+    c = slice(n // 2 - n // 8, n // 2 + n // 8)
+    u[c, c] = 0.5
+    v[c, c] = 0.25
+    u += rng.uniform(-0.02, 0.02, (n, n)).astype(np.float32)
+    v += rng.uniform(-0.02, 0.02, (n, n)).astype(np.float32)
+    terms = [("u", ["u", "v", "v"], [(-1.0,)]), ("u", [], [(F,)]), ("u", ["u"], [(-F,)]),
+             ("v", ["u", "v", "v"], [(1.0,)]), ("v", ["v"], [(-(F + k),)])]
+    return {"dims": (n, n, 1), "dt": dt, "fields": [("u", True), ("v", True)],
+            "implicit": {"u": [(-0.16, 1)], "v": [(-0.08, 1)]}, "terms": terms,
+            "init": {"u": u.ravel(), "v": v.ravel()}}

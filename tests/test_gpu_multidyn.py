@@ -28,6 +28,7 @@ import pytest
 
 import cupss_amd as C
 from oracle import oracle as O
+from cupss_amd.systems import spec_gray_scott
 from tests.helpers import build_system, rel_l2, spec_cahn_hilliard
 
 pytestmark = pytest.mark.gpu
@@ -40,23 +41,6 @@ def gpu(spec, path=0, noise_mode=0, profile=False):
     if profile:
         g.profile_kernel(True)
     return g, ids
-
-
-def spec_gray_scott(n=64, seed=31, F=0.035, k=0.065):
-    """dt u + 0.16 q^2 u = -u v^2 + F (1 - u);  dt v + 0.08 q^2 v = u v^2 - (F + k) v"""
-    rng = np.random.default_rng(seed)
-    u = np.ones((n, n), np.float32)
-    v = np.zeros((n, n), np.float32)
-    c = slice(n // 2 - n // 8, n // 2 + n // 8)
-    u[c, c] = 0.5
-    v[c, c] = 0.25
-    u += rng.uniform(-0.02, 0.02, (n, n)).astype(np.float32)
-    v += rng.uniform(-0.02, 0.02, (n, n)).astype(np.float32)
-    terms = [("u", ["u", "v", "v"], [(-1.0,)]), ("u", [], [(F,)]), ("u", ["u"], [(-F,)]),
-             ("v", ["u", "v", "v"], [(1.0,)]), ("v", ["v"], [(-(F + k),)])]
-    return {"dims": (n, n, 1), "dt": 0.5, "fields": [("u", True), ("v", True)],
-            "implicit": {"u": [(-0.16, 1)], "v": [(-0.08, 1)]}, "terms": terms,
-            "init": {"u": u.ravel(), "v": v.ravel()}}
 
 
 def spec_coupled(dims=(64, 32, 1), seed=32, noise=None):
