@@ -1,5 +1,5 @@
 """Serving (cupss_server_* through the Python Server binding): jobs of
-different sizes and models arrive over time, are admitted by memory budget
+different sizes and models (one or two dynamic fields) arrive over time, are admitted by memory budget
 and advance together with continuous batching.  Checked here: each job's
 result is bit-identical to advancing its system alone, admission keeps
 arrival order and the budget, and the general step program replayed from a
@@ -9,13 +9,14 @@ import numpy as np
 import pytest
 
 import cupss_amd as C
-from cupss_amd.systems import build, spec_cahn_hilliard, spec_kpz, spec_model_h
+from cupss_amd.systems import build, spec_cahn_hilliard, spec_gray_scott, spec_kpz, spec_model_h
 
 pytestmark = pytest.mark.gpu
 
 
 def job_specs():
-    # (name, spec, steps); the dynamic field is field 0 in every spec
+    # (name, spec, steps); field 0 is a dynamic field in every spec (the
+    # Gray-Scott jobs have two)
     return [
         ("modelh64", spec_model_h(64, seed=11), 37),
         ("kpz64", spec_kpz(64, nu=1.0, lam=1.5, seed=5), 50),
@@ -24,6 +25,8 @@ def job_specs():
         ("ew64", spec_kpz(64, nu=0.7, lam=0.0, seed=9), 44),
         ("modelh64b", spec_model_h(64, seed=13), 10),
         ("kpz128", spec_kpz(128, nu=1.3, lam=0.8, seed=6), 31),
+        ("gs64", spec_gray_scott(64, seed=15), 29),
+        ("gs128", spec_gray_scott(128, seed=16), 41),
     ]
 
 
