@@ -5,7 +5,8 @@ initializeFromFile / advanceTime / writeOut) on the HIP path.
   spectral operators, RUN_CPU and RUN_GPU) against the golden files.
 * tests/cpp/system_driver: systems read from reference-format system files,
   stepped through advanceTime and written with the reference CSV output,
-  against the oracle's FP32 run of the same parsed system.
+  against the oracle's FP32 run of the same parsed system (Cahn-Hilliard in
+  2D and 3D, model H, Gray-Scott with two dynamic fields).
 * tests/cpp/hooks_driver: field callbacks, the reference's float2 kind and
   the device-side kind, give the same fields (mirror boundary conditions of
   the reference's example 07; a Fourier-space low-pass filter).
@@ -17,6 +18,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
+from cupss_amd.systems import spec_gray_scott
 from tests.helpers import build_system, rel_l2, spec_cahn_hilliard, spec_model_h
 
 pytestmark = pytest.mark.gpu
@@ -122,3 +124,21 @@ def test_system_file_model_h(tmp_path):
             # unit of the last place = 5e-7): bound the pointwise deviation
             # by that quantisation plus 1e-4 of the field's scale
             assert dev <= 5e-7 + 1e-4 * float(np.max(np.abs(ref)))
+
+
+def test_system_file_gray_scott_two_dynamic_fields(tmp_path):
+    """Two dynamic fields through the parser and advanceTime (the fused step
+    advancing v as a stored dynamic field), against the oracle."""
+    spec = spec_gray_scott(64, seed=17)
+    lines = ["Fields", "u 1 1", "v 1 1", "Parameters", "F 0.035", "kap 0.065", "Du 0.16", "Dv 0.08", "Equations",
+             "dt u + Du*q^2*u = -u*v^2 + F - F*u", "dt v + Dv*q^2*v = u*v^2 - (F + kap)*v"]
+    out = driver(tmp_path, (64, 64, 1), 0.5, 100, lines, {"u": spec["init"]["u"], "v": spec["init"]["v"]})
+    sp = dict(spec, precision="f32")
+    # the parser keeps -(F + kap)*v as two prefactors of one term
+    sp["terms"] = [t if t[:2] != ("v", ["v"]) else ("v", ["v"], [(-0.035,), (-0.065,)]) for t in spec["terms"]]
+    o, ids = build_system(O.System, O.pres, sp)
+    o.step(100)
+    for f in ("u", "v"):
+        err = rel_l2(out(f), o.get_real(ids[f]))
+        print("system file Gray-Scott", f, "rel L2 vs oracle %.3e" % err)
+        assert err < 1e-4, f
