@@ -268,6 +268,7 @@ static int describe(cupss_system s, FusedInst &a, bool natural = false)
     }
     // stored constraint fields
     a.ncons = (int)P.cons.size();
+    a.nfresh = P.nfresh;
     // targets: phi, then the stored fields; groups are ordered the same way
     std::vector<int> targets = {P.phi};
     targets.insert(targets.end(), P.cons.begin(), P.cons.end());
@@ -301,9 +302,10 @@ static int describe(cupss_system s, FusedInst &a, bool natural = false)
             FT.pres0 = R.pres0;
             FT.npres = R.npres;
             FT.idx = 0;
-            // a dynamic target (phi, or a stored dynamic field) reads the
-            // values before the step, a constraint the dynamic fields' after it
-            const bool dyn_target = k == 0 || (int)k - 1 < P.nxdyn;
+            // a dynamic target (phi, or a stored dynamic field) and a fresh
+            // constraint read the values before the step, the other
+            // constraints the dynamic fields' after it
+            const bool dyn_target = k == 0 || (int)k - 1 < P.nxdyn + P.nfresh;
             if (T.fac.empty()) {
                 FT.src = SRC_CONST;
             } else if (T.fac[0] == P.phi) {
@@ -320,6 +322,7 @@ static int describe(cupss_system s, FusedInst &a, bool natural = false)
         TG.imp0 = np;
         TG.z = -1;
         TG.dyn = 0;
+        TG.fresh = (k > 0 && (int)k - 1 >= P.nxdyn && (int)k - 1 < P.nxdyn + P.nfresh) ? 1 : 0;
         TG.noisy = 0;
         if (k > 0 && (int)k - 1 < P.nxdyn) {
             // a further dynamic field: its noise and D(q) from the implicit
@@ -522,7 +525,7 @@ static int build(FusedBatch &B)
         }
         int stored = a.ncons > 0 ? 1 : 0;
         for (int k = 0; k < a.ncons; k++)
-            if (a.tgt[1 + k].dyn) stored = 2;
+            if (a.tgt[1 + k].dyn || a.tgt[1 + k].fresh) stored = 2;
         auto key = std::make_pair(a.g.nc * 8 + (a.nreal > 3 ? 4 : 0) + stored, a.g.sy);
         if (!cls.count(key)) {
             cls[key] = (int)B.classes.size();
@@ -890,7 +893,7 @@ static int run(FusedBatch &B, int nsteps)
     for (cupss_system s : B.sys) {
         s->resume_now = !will_tune && s->cont_step == s->step && s->cont_gen == s->gen;
         s->cont_now = !s->plan.cons.empty() && s->resume_now;
-        if ((int)s->plan.cons.size() > s->plan.nxdyn && !s->cont_now) {
+        if ((int)s->plan.cons.size() > s->plan.nxdyn + s->plan.nfresh && !s->cont_now) {
             if (int r = general_constraint_phase(s)) return r;
             general = true;
         }

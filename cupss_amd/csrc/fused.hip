@@ -368,6 +368,24 @@ __device__ __forceinline__ void pool_sum_col(const PresDev *pool, int p0, int n,
     }
 }
 
+// constraint value: the term sum divided by the implicit operator's sum
+// where q != 0 (setNotDynamic_k, src/field_kernels.cu:130-197)
+template <int E>
+__device__ __forceinline__ void constraint_solve(const FusedInst &a, const FusedTarget &TG, const ColBins<E> &cb,
+                                                 float2 (&v)[E])
+{
+    if (TG.nimp <= 0) return;
+    float lsum[E];
+    pool_sum_col<E>(a.pool, TG.imp0, TG.nimp, cb, lsum);
+#pragma unroll
+    for (int e = 0; e < E; e++)
+        if (!cb.zero[e]) {
+            const float rl = __builtin_amdgcn_rcpf(lsum[e]);
+            v[e].x *= rl;
+            v[e].y *= rl;
+        }
+}
+
 // 1/D(q), D = 1 - dt sum of the pool prefactors [p0, p0 + n) (the implicit
 // operator of a stored dynamic field, summed in implicit_col's order)
 template <int E>
@@ -473,7 +491,9 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
     float2 *L = lds + c * C::LL;
     const long col = (long)kx * N;
     const bool upd = mode == 1;
-    const bool pre = upd && a.tgt[0].g1 > a.tgt[0].g0;  // phi has product groups: load the first early
+    // phi has product groups: load the first early (unless fresh constraints,
+    // solved first, need the register block)
+    const bool pre = upd && a.tgt[0].g1 > a.tgt[0].g0 && !(XDYN && a.nfresh > 0);
     const float2 *__restrict__ sin = a.spec[parity];
 
     float2 cur[E], xv[E];
@@ -489,8 +509,28 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
     ColBins<E> cb;
     col_bins<E, T>(g, kx, t, cb);
     float2 nw[E];  // phi_{m+1}
+    // workgroups holding the column kx = 0 or kx = sx/2 (uniform test)
+    const bool edge_wg = blk.line == 0 || blk.line + TWD > g.nc;
+    const bool edge = active && (kx == 0 || kx == g.nc);
     if (upd) {
         const uint64_t step = a.step_base + step_rel;
+        // ---- fresh constraints (setNotDynamic_k of phase A, src/field_kernels.cu:
+        // 130-197): their values of step m, from the dynamic fields of step m,
+        // into the buffer the dynamic fields' terms read
+        if (XDYN && a.nfresh > 0)
+            for (int ci = 0; ci < a.ncons; ci++) {
+                const FusedTarget &TG = a.tgt[1 + ci];
+                if (!TG.fresh) continue;
+                float2 v[E];
+                target_sum<N, E, T>(a, TG, cb, cur, cur, xv, false, parity, active, col, L, t, tws, v);
+                constraint_solve<E>(a, TG, cb, v);
+                if (edge_wg) hermitian_col<N, E, T>(v, L, t, edge);
+                if (active) {
+                    float2 *__restrict__ cout = a.cs[ci][parity];
+#pragma unroll
+                    for (int e = 0; e < E; e++) gstore(cout, col + t + e * T, v[e]);
+                }
+            }
         // ---- phi (setDynamic_k, src/field_kernels.cu:199-227)
         float2 acc[E];
         target_sum<N, E, T>(a, a.tgt[0], cb, cur, cur, xv, pre, parity, active, col, L, t, tws, acc);
@@ -529,9 +569,6 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
             }
             nw[e] = s;
         }
-        // workgroups holding the column kx = 0 or kx = sx/2 (uniform test)
-        const bool edge_wg = blk.line == 0 || blk.line + TWD > g.nc;
-        const bool edge = active && (kx == 0 || kx == g.nc);
         if (edge_wg) hermitian_col<N, E, T>(nw, L, t, edge);
         if (active) {
             float2 *__restrict__ sout = a.spec[parity ^ 1];
@@ -544,6 +581,7 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
         // pre-step state of step m+1
         for (int ci = 0; ci < (CONS ? a.ncons : 0); ci++) {
             const FusedTarget &TG = a.tgt[1 + ci];
+            if (XDYN && TG.fresh) continue;
             float2 v[E];
             target_sum<N, E, T>(a, TG, cb, cur, nw, xv, false, parity, active, col, L, t, tws, v);
             if (XDYN && TG.dyn) {
@@ -585,16 +623,8 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
                         v[e].y *= rD[e];
                     }
                 }
-            } else if (TG.nimp > 0) {
-                float lsum[E];
-                pool_sum_col<E>(a.pool, TG.imp0, TG.nimp, cb, lsum);
-#pragma unroll
-                for (int e = 0; e < E; e++)
-                    if (!cb.zero[e]) {
-                        const float rl = __builtin_amdgcn_rcpf(lsum[e]);
-                        v[e].x *= rl;
-                        v[e].y *= rl;
-                    }
+            } else {
+                constraint_solve<E>(a, TG, cb, v);
             }
             if (edge_wg) hermitian_col<N, E, T>(v, L, t, edge);
             if (active) {

@@ -6,11 +6,13 @@
 //   derived:  psi = M_psi u for a dynamic field u (one single-field term, no
 //             implicit operator, no noise) -- never stored during a run,
 //             evaluated from u where needed;
-//   stored:   the other dynamic fields (2D), then any other constraint
-//             field without noise whose
-//             product terms read only derived or stored constraint fields --
-//             kept as state, updated every pass (the "phase A/B" of the next
-//             step, see fused.hip).
+//   stored:   the other dynamic fields (2D); constraints whose products
+//             read dynamic fields only and whose other terms are dynamic
+//             fields or constants ("fresh": solved at the start of a pass);
+//             then any other constraint field without noise whose product
+//             terms read only derived or stored constraint fields -- kept as
+//             state, updated every pass (the "phase A/B" of the next step,
+//             see fused.hip).  No product reads a fresh constraint.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -65,6 +67,8 @@ struct FusedTarget {
                      // dynamic: D(q) = 1 - dt sum)
     int z;           // stored field: index of its real field, -1 if none
     int dyn;         // stored dynamic field: semi-implicit Euler step of its own value
+    int fresh;       // stored constraint solved at the start of the pass from the dynamic
+                     // fields before the step (its products read dynamic fields)
     int noisy;       // stored dynamic field: its noise mode (NoiseMode), amplitude as UpdateArgs
     float noise_base;
     int noise_q2n, noise_invq;
@@ -101,6 +105,7 @@ struct FusedInst {
     // column pass
     UpdateArgs u;                             // phi: dt, noise, implicit D(q) (and its terms, elementwise kernel)
     int ncons;
+    int nfresh;                               // fresh constraints among the stored fields
     int cont;                                 // cs[.][0] already holds this run's first phase A/B
     int resume;                               // spec[0] and Z_f already hold this run's start (no prologue)
     FusedTarget tgt[FUSED_MAX_TARGETS];
@@ -146,7 +151,8 @@ hipError_t fused_rows_launch(int nc, int lpb, bool wide, const FusedInst *inst, 
 // wall-clock ticks, FUSED_TS_STRIDE apart, that the launch's workgroups
 // min/max-update; the launch spans min(starts) .. max(ends))
 // (stored: 0 = no instance of the launch has stored fields, 1 = some have
-// stored constraint fields, 2 = some have stored dynamic fields)
+// stored constraint fields, 2 = some have stored dynamic fields or fresh
+// constraints)
 hipError_t fused_cols_launch(int ny, int twd, int stored, const FusedInst *inst, const FusedBlock *blocks, int nblocks,
                              int parity, uint32_t step_rel, int mode, hipStream_t s, unsigned long long *ts = nullptr);
 // the slots of n timed launches to {~0, 0} (a kernel, so that graphs can hold it)

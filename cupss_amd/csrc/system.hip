@@ -892,10 +892,29 @@ static void plan_fused(cupss_system s)
             }
         }
         if (F.terms.empty()) return;  // keeps its value: the general program's case
+        stored[i] = 3;
+    }
+    // A constraint whose products read a dynamic field is "fresh" (kind 2):
+    // the reference's phase A of step m reads the dynamic fields of step m,
+    // so it is solved at the start of the pass that advances them, from
+    // products of dynamic fields only and single terms of dynamic fields or
+    // constants (what that pass holds); nothing reads it in real space.
+    // The other constraints (kind 3) are solved for the next step.
+    for (int i = 0; i < nf; i++) {
+        if (stored[i] != 3) continue;
+        bool reads_dyn = false;
+        for (int ti : s->f[i].terms)
+            if (s->t[ti].fac.size() >= 2)
+                for (int c : s->t[ti].fac) reads_dyn = reads_dyn || dyn[c];
+        if (!reads_dyn) continue;
+        for (int ti : s->f[i].terms)
+            for (int c : s->t[ti].fac)
+                if (!dyn[c]) return;
         stored[i] = 2;
     }
-    // product factors: dynamic fields' products read anything; stored
-    // constraints' products read derived or stored constraint fields only
+    // product factors: dynamic fields' products read anything but fresh
+    // constraints; the other constraints' products read derived or kind-3
+    // constraint fields only
     std::vector<std::vector<int>> prods_of(nf);
     for (int i = 0; i < nf; i++) {
         if (derived[i]) continue;
@@ -905,7 +924,8 @@ static void plan_fused(cupss_system s)
             if (T.fac.size() >= 2) {
                 if ((int)T.fac.size() > 8) return;
                 for (int c : T.fac) {
-                    if (!dyn[i] && dyn[c]) return;
+                    if (stored[c] == 2) return;
+                    if (stored[i] == 3 && dyn[c]) return;
                     if (std::find(P.real.begin(), P.real.end(), c) == P.real.end()) P.real.push_back(c);
                 }
                 prods_of[i].push_back(ti);
@@ -917,11 +937,13 @@ static void plan_fused(cupss_system s)
     }
     if ((int)P.real.size() > FUSED_MAX_REAL) return;
     if (!fused_shapes_exist(s->g.sx, s->g.sy, s->g.sz, (int)P.real.size())) return;
-    for (int kind = 1; kind <= 2; kind++)
+    for (int kind = 1; kind <= 3; kind++)
         for (int i = 0; i < nf; i++)
             if (stored[i] == kind) P.cons.push_back(i);
-    for (int i = 0; i < nf; i++)
+    for (int i = 0; i < nf; i++) {
         if (stored[i] == 1) P.nxdyn++;
+        if (stored[i] == 2) P.nfresh++;
+    }
     if ((int)P.cons.size() > FUSED_MAX_CONS) return;
     if (s->g.sz > 1 && !P.cons.empty()) return;  // the 3D step has no stored fields
     // groups: phi's first, then each stored field's
@@ -997,7 +1019,7 @@ static void plan_fused(cupss_system s)
 static bool same_plan_shape(const FusedPlan &a, const FusedPlan &b)
 {
     if (a.ok != b.ok || a.phi != b.phi || a.real != b.real || a.derived != b.derived || a.dbase != b.dbase ||
-        a.cons != b.cons || a.nxdyn != b.nxdyn || a.groups.size() != b.groups.size())
+        a.cons != b.cons || a.nxdyn != b.nxdyn || a.nfresh != b.nfresh || a.groups.size() != b.groups.size())
         return false;
     for (size_t i = 0; i < a.groups.size(); i++) {
         const FusedGroup &x = a.groups[i], &y = b.groups[i];

@@ -69,8 +69,10 @@ struct FusedPlan {
     std::vector<FusedGroup> groups;   // product groups, phi's first, then by stored field
     std::vector<int> derived;         // constraint fields psi = M_psi u, u dynamic
     std::vector<int> dbase;           // u of each derived field
-    std::vector<int> cons;            // stored fields: the other dynamic fields (nxdyn), then constraints
+    std::vector<int> cons;            // stored fields: the other dynamic fields (nxdyn), the fresh
+                                      // constraints (nfresh), then the other constraints
     int nxdyn = 0;
+    int nfresh = 0;
 };
 
 }  // namespace cupss

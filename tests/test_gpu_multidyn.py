@@ -22,6 +22,12 @@ state.
 * A grid whose row kernel does not exist for the number of real fields
   (more than 3 at sx = 32) takes the general program and still matches the
   oracle.
+* Constraints whose products read dynamic fields ("fresh": the reference's
+  phase A of step m reads the dynamic fields of step m, so the column pass
+  solves them before the dynamic update): Cahn-Hilliard written with its
+  chemical potential mu = b phi^3 against the oracle and against the usual
+  form, and a two-field system where such a constraint is read by both
+  dynamic fields and, one step late, by another constraint.
 """
 import numpy as np
 import pytest
@@ -151,3 +157,85 @@ def test_no_row_kernel_for_the_grid_takes_the_general_program():
         err = rel_l2(g.get_real(gid[f]), o.get_real(oid[f]))
         print("  32x64, %-3s general-vs-f32 %.3e" % (f, err), flush=True)
         assert err <= 1e-4, f
+
+
+def spec_mu_form(dims=(128, 64, 1), seed=45, a=-1.0, b=1.0, k=1.0):
+    """dt phi + (a q^2 + k q^4) phi = -q^2 mu;   mu = b phi^3"""
+    n = dims[0] * dims[1] * dims[2]
+    x0 = np.random.default_rng(seed).uniform(-0.1, 0.1, n).astype(np.float32)
+    return {"dims": dims, "dt": 0.01, "fields": [("phi", True), ("mu", False)],
+            "implicit": {"phi": [(-a, 1), (-k, 2)]},
+            "terms": [("phi", ["mu"], [(-1.0, 1)]), ("mu", ["phi", "phi", "phi"], [(b,)])],
+            "init": {"phi": x0}}
+
+
+def spec_fresh_coupled(dims=(64, 64, 1), seed=46):
+    """dt phi + q^2 phi + q^4 phi = -q^2 mu
+       dt psi + 0.5 q^2 psi = 0.2 mu + 0.1 s psi
+       mu = phi^3 + 0.3 phi psi + 0.1 psi - 0.05   (fresh: products of dynamic fields)
+       s = 0.5 mu                                 (reads mu one step late)"""
+    n = dims[0] * dims[1] * dims[2]
+    rng = np.random.default_rng(seed)
+    terms = [("phi", ["mu"], [(-1.0, 1)]),
+             ("psi", ["mu"], [(0.2,)]), ("psi", ["s", "psi"], [(0.1,)]),
+             ("mu", ["phi", "phi", "phi"], [(1.0,)]), ("mu", ["phi", "psi"], [(0.3,)]), ("mu", ["psi"], [(0.1,)]),
+             ("mu", [], [(-0.05,)]),
+             ("s", ["mu"], [(0.5,)])]
+    return {"dims": dims, "dt": 0.01, "fields": [("phi", True), ("psi", True), ("mu", False), ("s", False)],
+            "implicit": {"phi": [(1.0, 1), (-1.0, 2)], "psi": [(-0.5, 1)]}, "terms": terms,
+            "init": {"phi": rng.uniform(-0.1, 0.1, n).astype(np.float32),
+                     "psi": rng.uniform(-0.1, 0.1, n).astype(np.float32)}}
+
+
+FRESH = {"mu_form": (spec_mu_form, ("phi", "mu")), "fresh_coupled": (spec_fresh_coupled, ("phi", "psi", "mu", "s"))}
+
+
+@pytest.mark.parametrize("case", sorted(FRESH))
+def test_fresh_constraints_fused_vs_oracle(case):
+    mk, names = FRESH[case]
+    spec = mk()
+    print("constraints of the dynamic fields' products, fused vs oracle:", case, flush=True)
+    g, gid = gpu(spec, profile=True)
+    o, oid = build_system(O.System, O.pres, dict(spec, precision="f32"))
+    g.step(100)
+    o.step(100)
+    assert_fused_stored_dynamic(g)
+    for f in names:
+        got, ref = g.get_real(gid[f]), o.get_real(oid[f])
+        assert np.all(np.isfinite(got))
+        err = rel_l2(got, ref)
+        print("  %-3s gpu-vs-f32 %.3e" % (f, err), flush=True)
+        assert err <= 1e-4, f
+    if case == "mu_form":
+        # the same scheme as the usual form: dt phi + (a q^2 + k q^4) phi = -b q^2 phi^3
+        u, uid = gpu(spec_cahn_hilliard(spec["dims"], seed=45))
+        u.set_real(uid["phi"], spec["init"]["phi"])
+        u.step(100)
+        err = rel_l2(g.get_real(gid["phi"]), u.get_real(uid["phi"]))
+        print("  mu form vs usual form %.3e" % err, flush=True)
+        assert err <= 1e-5
+
+
+@pytest.mark.parametrize("case", sorted(FRESH))
+def test_fresh_constraints_split_runs_and_batch_bitwise(case):
+    mk, names = FRESH[case]
+    spec = mk()
+    one, ia = gpu(spec)
+    one.step(30)
+    split, ib = gpu(spec)
+    for n in (7, 1, 12, 10):
+        split.step(n)
+    for f in names:
+        assert np.array_equal(one.get_real(ia[f]), split.get_real(ib[f])), f
+    batch = C.Batch(graph_steps=4)
+    members = []
+    for sp in (spec, spec_gray_scott(64, seed=47), spec_cahn_hilliard((64, 64, 1), seed=48)):
+        g, ids = gpu(sp)
+        batch.add(g)
+        members.append((g, ids, sp))
+    batch.step(30)
+    for g, ids, sp in members:
+        alone, ia2 = gpu(sp)
+        alone.step(30)
+        for f in ids:
+            assert np.array_equal(alone.get_real(ia2[f]), g.get_real(ids[f])), f
