@@ -323,6 +323,7 @@ static int describe(cupss_system s, FusedInst &a, bool natural = false)
         TG.z = -1;
         TG.dyn = 0;
         TG.fresh = (k > 0 && (int)k - 1 >= P.nxdyn && (int)k - 1 < P.nxdyn + P.nfresh) ? 1 : 0;
+        TG.keep = (k > 0 && (int)k - 1 >= P.nxdyn && s->f[f].terms.empty()) ? 1 : 0;
         TG.noisy = 0;
         if (k > 0 && (int)k - 1 < P.nxdyn) {
             // a further dynamic field: its noise and D(q) from the implicit

@@ -623,6 +623,11 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
                         v[e].y *= rD[e];
                     }
                 }
+            } else if (TG.keep) {
+                // no terms: setNotDynamic_k leaves the value as it is
+                const float2 *__restrict__ kin = a.cs[ci][parity];
+#pragma unroll
+                for (int e = 0; e < E; e++) v[e] = active ? gload(kin, col + t + e * T) : make_float2(0.f, 0.f);
             } else {
                 constraint_solve<E>(a, TG, cb, v);
             }

@@ -10,7 +10,8 @@
 //             read dynamic fields only and whose other terms are dynamic
 //             fields or constants ("fresh": solved at the start of a pass);
 //             then any other constraint field without noise whose product
-//             terms read only derived or stored constraint fields -- kept as
+//             terms read only derived or stored constraint fields (or that
+//             has no terms and no implicit operator: it keeps its value) -- kept as
 //             state, updated every pass (the "phase A/B" of the next step,
 //             see fused.hip).  No product reads a fresh constraint.
 #pragma once
@@ -69,6 +70,7 @@ struct FusedTarget {
     int dyn;         // stored dynamic field: semi-implicit Euler step of its own value
     int fresh;       // stored constraint solved at the start of the pass from the dynamic
                      // fields before the step (its products read dynamic fields)
+    int keep;        // stored constraint without terms: keeps its value (a coefficient field)
     int noisy;       // stored dynamic field: its noise mode (NoiseMode), amplitude as UpdateArgs
     float noise_base;
     int noise_q2n, noise_invq;

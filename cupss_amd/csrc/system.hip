@@ -891,7 +891,10 @@ static void plan_fused(cupss_system s)
                 continue;
             }
         }
-        if (F.terms.empty()) return;  // keeps its value: the general program's case
+        // no terms and no implicit operator: it keeps its value (a coefficient
+        // field); with an implicit operator setNotDynamic_k would divide the
+        // kept value every step, which the general program reproduces
+        if (F.terms.empty() && !F.imp.empty()) return;
         stored[i] = 3;
     }
     // A constraint whose products read a dynamic field is "fresh" (kind 2):

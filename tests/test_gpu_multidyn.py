@@ -28,6 +28,9 @@ state.
   chemical potential mu = b phi^3 against the oracle and against the usual
   form, and a two-field system where such a constraint is read by both
   dynamic fields and, one step late, by another constraint.
+* A constraint field without terms keeps its value (a coefficient field;
+  setNotDynamic_k writes nothing): example 07's inhomogeneous diffusion
+  without its callbacks, against the oracle.
 """
 import numpy as np
 import pytest
@@ -239,3 +242,31 @@ def test_fresh_constraints_split_runs_and_batch_bitwise(case):
         alone.step(30)
         for f in ids:
             assert np.array_equal(alone.get_real(ia2[f]), g.get_real(ids[f])), f
+
+
+def spec_inhomogeneous_diffusion(n=128, seed=49):
+    """Example 07's system (examples/07_inhomogeneous_diffusion/
+    07_inhomogeneous_diffusion.cu:62-81) without the mirror callbacks:
+    dt v + 0.5 q^2 v = iqx (x iqxv) + iqy^2 (x v);  iqxv = iqx v;  x fixed."""
+    i = np.arange(n, dtype=np.float64)
+    x = np.tile(1.0 + np.tanh((i - 0.45 * n) / (0.05 * n)), (n, 1)).astype(np.float32)
+    v = np.random.default_rng(seed).uniform(0.0, 1.0, (n, n)).astype(np.float32)
+    terms = [("v", ["x", "iqxv"], [(1.0, 0, 1)]), ("v", ["x", "v"], [(1.0, 0, 0, 2)]),
+             ("iqxv", ["v"], [(1.0, 0, 1)])]
+    return {"dims": (n, n, 1), "dt": 0.01, "fields": [("v", True), ("iqxv", False), ("x", False)],
+            "implicit": {"v": [(-0.5, 1)]}, "terms": terms, "init": {"v": v.ravel(), "x": x.ravel()}}
+
+
+def test_coefficient_field_keeps_its_value():
+    spec = spec_inhomogeneous_diffusion()
+    g, gid = gpu(spec, profile=True)
+    o, oid = build_system(O.System, O.pres, dict(spec, precision="f32"))
+    g.step(100)
+    o.step(100)
+    name = g.kernel_time()[2]
+    assert name.startswith("k_fcols<") and name.endswith("true, false>"), name
+    for f in ("v", "iqxv", "x"):
+        err = rel_l2(g.get_real(gid[f]), o.get_real(oid[f]))
+        print("  %-4s gpu-vs-f32 %.3e" % (f, err), flush=True)
+        assert err <= 1e-4, f
+    assert rel_l2(g.get_real(gid["x"]), spec["init"]["x"]) <= 5e-6  # R2C/C2R round trips only
