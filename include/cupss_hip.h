@@ -151,10 +151,16 @@ int cupss_system_get_step(cupss_system s, uint64_t *step);
 int cupss_system_set_step(cupss_system s, uint64_t step);
 /* use the caller's hipStream_t (null = the library's own stream) */
 int cupss_system_set_stream(cupss_system s, void *hip_stream);
-/* Step implementation: 0 (default) = the fused two-kernel 2D step when the
- * system qualifies (one dynamic field; other fields linear in it), else the
- * general step program; 1 = always the general step program.  Both run on
- * the GPU and follow the same semantics. */
+/* Step implementation: 0 (default) = a fused step when the system
+ * qualifies, else the general step program; 1 = always the general step
+ * program.  Both run on the GPU and follow the same semantics.  The fused
+ * two-kernel 2D step takes any number of dynamic fields with fields
+ * linear in one of them, constraint fields of products (of dynamic fields,
+ * or of derived and other such constraint fields) and coefficient fields
+ * without terms; the fused 3D step takes one dynamic field with fields
+ * linear in it and spectral-space noise.  Neither takes field callbacks
+ * (either kind), noisy constraint fields or grids outside 16..4096 x
+ * 8..4096 (x 8..1024). */
 int cupss_system_set_path(cupss_system s, int mode);
 /* Noise draw: 0 (default) = xi ~ N(0,1) per real-space cell, then R2C (the
  * reference's createNoise, src/field.cpp:300-316; the oracle reproduces it
@@ -162,7 +168,8 @@ int cupss_system_set_path(cupss_system s, int mode);
  * law (complex Gaussian, E|eta|^2 = N, Hermitian), which saves the
  * transform.  Not in the reference. */
 int cupss_system_set_noise_mode(cupss_system s, int mode);
-/* 1 if cupss_system_advance would take the fused step */
+/* 1 if cupss_system_advance would take the fused 2D step (and
+ * cupss_batch_add accept the system) */
 int cupss_system_fused_eligible(cupss_system s);
 /* 0 = direct launches (default); K (even, >= 2) = runs of at least K steps
  * replay K steps from a hipGraph captured at the first such run.  Applies
@@ -183,8 +190,8 @@ int cupss_system_set_dt(cupss_system s, float dt);
  * Many independent systems advanced together: per step, one row and one
  * column launch per size class of systems (and one elementwise launch for
  * systems that need no transform), optionally replayed from a hipGraph.
- * Member systems must be prepared, qualify for the fused step (one dynamic
- * field, other fields linear in it, no callbacks) and must not be advanced
+ * Member systems must be prepared 2D systems that qualify for the fused
+ * step (cupss_system_fused_eligible; no callbacks) and must not be advanced
  * on their own while in the batch.  Each keeps its own parameters, step
  * counter and noise stream, so a system advanced in a batch follows the
  * same sequence as when advanced alone.  No counterpart in the reference
