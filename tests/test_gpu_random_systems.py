@@ -18,6 +18,11 @@ and checks every field against the oracle (relative L2 <= 1e-4, or absolute
 planner's classification is what gets tested.  Cases that draw a shape the
 fused kernels do not take (more than 3 real fields at sx = 32) run on the
 general program and are checked the same way.
+
+A second test advances 24 of the fused cases together in one batch.  Their
+size classes mix row-kernel widths and the three column-kernel variants
+(no stored fields, stored constraints, stored dynamic or fresh fields).
+Each member must be bit-identical to the same system advanced alone.
 """
 import numpy as np
 import pytest
@@ -117,3 +122,30 @@ def test_random_systems_against_oracle():
         g.close()
     print("fused-eligible cases: %d of %d" % (fused, len(SEEDS)), flush=True)
     assert fused >= len(SEEDS) // 2
+
+
+def test_random_systems_batched_equal_alone():
+    specs = []
+    for seed in SEEDS:
+        spec = random_spec(1000 + seed)
+        g, _ = build_system(C.System, C.pres, spec)
+        ok = g.fused_eligible
+        g.close()
+        if ok:
+            specs.append((1000 + seed, spec))
+        if len(specs) == 24:
+            break
+    batch = C.Batch(graph_steps=4)
+    members = []
+    for seed, sp in specs:
+        g, ids = build_system(C.System, C.pres, sp)
+        batch.add(g)
+        members.append((seed, g, ids, sp))
+    print("batch of %d random systems" % len(members), flush=True)
+    batch.step(18)
+    for seed, g, ids, sp in members:
+        a, ia = build_system(C.System, C.pres, sp)
+        a.step(18)
+        for f in ids:
+            assert np.array_equal(a.get_real(ia[f]), g.get_real(ids[f])), (seed, f)
+        a.close()
