@@ -23,6 +23,10 @@ A second test advances 24 of the fused cases together in one batch.  Their
 size classes mix row-kernel widths and the three column-kernel variants
 (no stored fields, stored constraints, stored dynamic or fresh fields).
 Each member must be bit-identical to the same system advanced alone.
+
+A third test adds noise to random dynamic fields of 30 cases and compares
+the fused step with the general program, which draws from the same Philox
+counters, in both noise modes (relative L2 <= 1e-5).
 """
 import numpy as np
 import pytest
@@ -149,3 +153,36 @@ def test_random_systems_batched_equal_alone():
         for f in ids:
             assert np.array_equal(a.get_real(ia[f]), g.get_real(ids[f])), (seed, f)
         a.close()
+
+
+@pytest.mark.parametrize("noise_mode", [0, 1])
+def test_random_noisy_systems_fused_equal_general(noise_mode):
+    checked = 0
+    for seed in SEEDS[:30]:
+        spec = random_spec(1000 + seed)
+        rng = np.random.default_rng(5000 + seed)
+        dyn = [f for f, d in spec["fields"] if d]
+        spec["noise"] = {d: ((float(rng.uniform(0.001, 0.01)),), 7000 + 10 * seed + i)
+                         for i, d in enumerate(dyn) if rng.random() < 0.6}
+        if not spec["noise"]:
+            continue
+        out = []
+        for path in (0, 1):
+            g, ids = build_system(C.System, C.pres, spec)
+            g.set_path(path)
+            g.set_noise_mode(noise_mode)
+            if path == 0 and not g.fused_eligible:
+                g.close()
+                break
+            g.step(20)
+            out.append({f: g.get_real(ids[f]) for f in ids})
+            g.close()
+        if len(out) < 2:
+            continue
+        checked += 1
+        for f in out[0]:
+            a, b = out[0][f], out[1][f]
+            err, scale = float(np.linalg.norm(a - b)), float(np.linalg.norm(b))
+            assert err <= 1e-5 * scale or err / np.sqrt(b.size) <= 1e-7, (1000 + seed, f, err, scale)
+    print("noise mode %d: %d noisy fused cases equal to the general program" % (noise_mode, checked), flush=True)
+    assert checked >= 10
