@@ -1,9 +1,12 @@
-// fused.hip -- the fused two-kernel step for 2D systems with one dynamic
-// field phi whose other fields are "derived": constraint fields
-// psi = M_psi(q) phi with no implicit operator and no noise (e.g. iqxh =
-// iqx*h).  Covers Cahn-Hilliard, Edwards-Wilkinson and KPZ.  Every launch
-// serves any number of instances (systems) of one size class through a
-// table of instance descriptors (FusedInst) and a per-workgroup block table.
+// fused.hip -- the fused two-kernel step for 2D systems: the first dynamic
+// field phi, fields linear in a dynamic field ("derived": psi = M_psi(q) u
+// with no implicit operator and no noise, e.g. iqxh = iqx*h) and stored
+// fields -- further dynamic fields, constraints of dynamic-field products,
+// other constraints and coefficient fields (fused.hpp).  Covers diffusion,
+// Cahn-Hilliard, Edwards-Wilkinson, KPZ, model H and reaction-diffusion
+// systems.  Every launch serves any number of instances (systems) of one
+// size class through a table of instance descriptors (FusedInst) and a
+// per-workgroup block table.
 //
 // Pipeline state between the kernels, all column-major (element (kx, y) at
 // kx*sy + y) so that the column kernel streams whole contiguous columns:
@@ -18,8 +21,9 @@
 //                  (runs of LPB*8 contiguous bytes per column), C2R along x
 //                  -> real dealiased fields -> products, noise -R2C-> X_g,
 //                  eta rows, written back through the same tile
-//   column kernel: X_g, eta columns -FFT_y-> spectral terms; update of
-//                  phi (setDynamic_k); mask_f * L_f * phi_new -IFFT_y-> Z_f
+//   column kernel: X_g, eta columns -FFT_y-> spectral terms; constraints of
+//                  dynamic-field products; update of phi (setDynamic_k);
+//                  the stored fields; mask_f * L_f * u_new -IFFT_y-> Z_f
 // Terms with equal prefactor lists form one group: M*FFT(a) + M*FFT(b) =
 // M*FFT(a + b).  The order of phases A-D of the reference
 // (src/evolver.cpp:199-226) is kept: derived fields are evaluated from phi
