@@ -323,24 +323,26 @@ static int describe(cupss_system s, FusedInst &a, bool natural = false)
         TG.z = -1;
         TG.dyn = 0;
         TG.fresh = (k > 0 && (int)k - 1 >= P.nxdyn && (int)k - 1 < P.nxdyn + P.nfresh) ? 1 : 0;
-        TG.keep = (k > 0 && (int)k - 1 >= P.nxdyn && s->f[f].terms.empty()) ? 1 : 0;
+        TG.keep = (k > 0 && (int)k - 1 >= P.nxdyn && s->f[f].terms.empty() && !s->f[f].noisy) ? 1 : 0;
         TG.noisy = 0;
-        if (k > 0 && (int)k - 1 < P.nxdyn) {
-            // a further dynamic field: its noise and D(q) from the implicit
-            // prefactors below
+        if (k > 0 && s->f[f].noisy) {
+            // a stored field's noise: its own key, stream and buffer; a
+            // constraint solved for the next step draws that step's noise
             UpdateArgs U;
             build_update(s, f, U);
             const FieldRec &G = s->f[f];
-            TG.dyn = 1;
-            TG.noisy = !G.noisy ? NOISE_NONE : (s->noise_mode == 1 ? NOISE_SPECTRAL : NOISE_REAL);
+            TG.noisy = s->noise_mode == 1 ? NOISE_SPECTRAL : NOISE_REAL;
             TG.eta = G.eta;
             TG.noise_base = U.noise_base;
             TG.noise_q2n = U.noise_q2n;
             TG.noise_invq = U.noise_invq;
+            TG.sdt = U.sdt;
             TG.key0 = (uint32_t)G.seed;
             TG.key1 = (uint32_t)(G.seed >> 32);
             TG.stream = (uint32_t)f;
+            TG.step_off = (int)k - 1 >= P.nxdyn + P.nfresh ? 1 : 0;
         }
+        if (k > 0 && (int)k - 1 < P.nxdyn) TG.dyn = 1;  // a further dynamic field: D(q) from its implicit prefactors
         if (k > 0) {
             for (const cupss_pres &q : s->f[f].imp) {
                 if (np >= FUSED_MAX_POOL) return fail(CUPSS_ERR_UNSUPPORTED, "too many prefactors for the fused step");
@@ -442,7 +444,7 @@ static double inst_bytes(const FusedInst &a)
 {
     int arrays = 2 + 2 * a.ncons + a.nreal + a.ngroups + (a.noise_mode == NOISE_REAL ? 1 : 0);
     for (int c = 0; c < a.ncons; c++)
-        if (a.tgt[1 + c].dyn && a.tgt[1 + c].noisy == NOISE_REAL) arrays++;
+        if (a.tgt[1 + c].noisy == NOISE_REAL) arrays++;
     return 8.0 * (double)a.g.nh * arrays;
 }
 
@@ -526,7 +528,7 @@ static int build(FusedBatch &B)
         }
         int stored = a.ncons > 0 ? 1 : 0;
         for (int k = 0; k < a.ncons; k++)
-            if (a.tgt[1 + k].dyn || a.tgt[1 + k].fresh) stored = 2;
+            if (a.tgt[1 + k].dyn || a.tgt[1 + k].fresh || a.tgt[1 + k].noisy) stored = 2;
         auto key = std::make_pair(a.g.nc * 8 + (a.nreal > 3 ? 4 : 0) + stored, a.g.sy);
         if (!cls.count(key)) {
             cls[key] = (int)B.classes.size();

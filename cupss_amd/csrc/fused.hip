@@ -230,7 +230,7 @@ __global__ __launch_bounds__((FRowCfg<NC, LPB, MAXR>::THREADS)) void k_frows(con
             eta = a.eta;
         } else {
             const FusedTarget &TG = a.tgt[1 + src];
-            if (!TG.dyn || TG.noisy != NOISE_REAL) continue;
+            if (TG.noisy != NOISE_REAL) continue;
             stream = TG.stream;
             key0 = TG.key0;
             key1 = TG.key1;
@@ -241,7 +241,7 @@ __global__ __launch_bounds__((FRowCfg<NC, LPB, MAXR>::THREADS)) void k_frows(con
         // t^1 (T is even): the even lane computes the calls of elements
         // e < E/2, the odd lane those of e >= E/2, and each hands its
         // partner the Box-Muller pair that the partner's cells take.
-        const uint64_t step = a.step_base + step_rel;
+        const uint64_t step = a.step_base + step_rel + (src < 0 ? 0 : a.tgt[1 + src].step_off);
         const uint32_t rowbase = (uint32_t)((long)(y0 + l) * NC);
         const bool even = (t & 1) == 0;
         float2 v[E];
@@ -369,6 +369,40 @@ __device__ __forceinline__ void pool_sum_col(const PresDev *pool, int p0, int n,
         }
 #pragma unroll
         for (int e = 0; e < E; e++) sum[e] += th[e];
+    }
+}
+
+// noise of a stored field over the E bins of a lane, times its amplitude
+// (times 1/sqrt(dt) for a constraint): real-space rows transformed along y,
+// or drawn in spectral space, at the pass's step + step_off
+template <int N, int E, int T>
+__device__ __forceinline__ void stored_noise(const FusedInst &a, const FusedTarget &TG, const ColBins<E> &cb, int kx,
+                                             uint64_t step, bool active, long col, float2 *L, int t,
+                                             const float2 *tws, float2 (&v)[E])
+{
+    float2 ue[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) ue[e] = make_float2(0.f, 0.f);
+    if (TG.noisy == NOISE_REAL) {
+#pragma unroll
+        for (int e = 0; e < E; e++) ue[e] = active ? gload(TG.eta, col + t + e * T) : make_float2(0.f, 0.f);
+        fft_line<N, E, -1, N>(ue, L, t, tws);
+        __syncthreads();
+    } else {
+        const uint64_t s = step + (uint64_t)TG.step_off;
+#pragma unroll
+        for (int e = 0; e < E / 2; e++)
+            spectral_noise_pair(a.g, kx, t + e * T, 0, (uint32_t)s, (uint32_t)(s >> 32), TG.stream, TG.key0, TG.key1,
+                                ue[e], ue[e + E / 2]);
+    }
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        float pn = TG.noise_base;
+        if (TG.noise_q2n != 0) pn *= sqrtf(ipow(cb.q2[e], TG.noise_q2n));
+        if (TG.noise_invq != 0) pn *= sqrtf(ipow(col_iq(cb, e), TG.noise_invq));
+        if (!TG.dyn) pn = TG.sdt * pn;
+        v[e].x += pn * ue[e].x;
+        v[e].y += pn * ue[e].y;
     }
 }
 
@@ -527,6 +561,7 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
                 if (!TG.fresh) continue;
                 float2 v[E];
                 target_sum<N, E, T>(a, TG, cb, cur, cur, xv, false, parity, active, col, L, t, tws, v);
+                if (TG.noisy) stored_noise<N, E, T>(a, TG, cb, kx, step, active, col, L, t, tws, v);
                 constraint_solve<E>(a, TG, cb, v);
                 if (edge_wg) hermitian_col<N, E, T>(v, L, t, edge);
                 if (active) {
@@ -590,34 +625,12 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
             target_sum<N, E, T>(a, TG, cb, cur, nw, xv, false, parity, active, col, L, t, tws, v);
             if (XDYN && TG.dyn) {
                 const float2 *__restrict__ uin = a.cs[ci][parity];
-                float2 ue[E];
 #pragma unroll
                 for (int e = 0; e < E; e++) {
-                    ue[e] = make_float2(0.f, 0.f);
                     const float2 u0 = active ? gload(uin, col + t + e * T) : make_float2(0.f, 0.f);
                     v[e] = make_float2(u0.x + a.u.dt * v[e].x, u0.y + a.u.dt * v[e].y);
                 }
-                if (TG.noisy == NOISE_REAL) {
-#pragma unroll
-                    for (int e = 0; e < E; e++) ue[e] = active ? gload(TG.eta, col + t + e * T) : make_float2(0.f, 0.f);
-                    fft_line<N, E, -1, N>(ue, L, t, tws);
-                    __syncthreads();
-                } else if (TG.noisy == NOISE_SPECTRAL) {
-#pragma unroll
-                    for (int e = 0; e < E / 2; e++)
-                        spectral_noise_pair(g, kx, t + e * T, 0, (uint32_t)step, (uint32_t)(step >> 32), TG.stream,
-                                            TG.key0, TG.key1, ue[e], ue[e + E / 2]);
-                }
-                if (TG.noisy) {
-#pragma unroll
-                    for (int e = 0; e < E; e++) {
-                        float pn = TG.noise_base;
-                        if (TG.noise_q2n != 0) pn *= sqrtf(ipow(cb.q2[e], TG.noise_q2n));
-                        if (TG.noise_invq != 0) pn *= sqrtf(ipow(col_iq(cb, e), TG.noise_invq));
-                        v[e].x += pn * ue[e].x;
-                        v[e].y += pn * ue[e].y;
-                    }
-                }
+                if (TG.noisy) stored_noise<N, E, T>(a, TG, cb, kx, step, active, col, L, t, tws, v);
                 if (TG.nimp > 0) {
                     float rD[E];
                     pool_implicit_col<E>(a.pool, TG.imp0, TG.nimp, a.u.dt, cb, rD);
@@ -633,6 +646,7 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
 #pragma unroll
                 for (int e = 0; e < E; e++) v[e] = active ? gload(kin, col + t + e * T) : make_float2(0.f, 0.f);
             } else {
+                if (XDYN && TG.noisy) stored_noise<N, E, T>(a, TG, cb, kx, step, active, col, L, t, tws, v);
                 constraint_solve<E>(a, TG, cb, v);
             }
             if (edge_wg) hermitian_col<N, E, T>(v, L, t, edge);

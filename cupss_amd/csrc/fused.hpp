@@ -9,7 +9,7 @@
 //   stored:   the other dynamic fields (2D); constraints whose products
 //             read dynamic fields only and whose other terms are dynamic
 //             fields or constants ("fresh": solved at the start of a pass);
-//             then any other constraint field without noise whose product
+//             then any other constraint field whose product
 //             terms read only derived or stored constraint fields (or that
 //             has no terms and no implicit operator: it keeps its value) -- kept as
 //             state, updated every pass (the "phase A/B" of the next step,
@@ -71,9 +71,12 @@ struct FusedTarget {
     int fresh;       // stored constraint solved at the start of the pass from the dynamic
                      // fields before the step (its products read dynamic fields)
     int keep;        // stored constraint without terms: keeps its value (a coefficient field)
-    int noisy;       // stored dynamic field: its noise mode (NoiseMode), amplitude as UpdateArgs
+    int noisy;       // stored field: its noise mode (NoiseMode), amplitude as UpdateArgs
     float noise_base;
     int noise_q2n, noise_invq;
+    float sdt;       // constraint: 1/sqrt(dt) on the amplitude (setNotDynamic_k)
+    int step_off;    // step counter of its noise: the pass's step + step_off (1: a
+                     // constraint solved for the next step)
     uint32_t key0, key1, stream;
     float2 *eta;     // NOISE_REAL: R2C_x(xi) rows, column-major (row pass)
 };

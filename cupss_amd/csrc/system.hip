@@ -882,8 +882,7 @@ static void plan_fused(cupss_system s)
             continue;
         }
         const FieldRec &F = s->f[i];
-        if (F.noisy) return;
-        if (F.imp.empty() && F.terms.size() == 1) {
+        if (!F.noisy && F.imp.empty() && F.terms.size() == 1) {
             const TermRec &T = s->t[F.terms[0]];
             if (T.fac.size() == 1 && dyn[T.fac[0]] && (int)T.pres.size() <= FUSED_MAX_POOL / 8) {
                 derived[i] = 1;
@@ -895,6 +894,7 @@ static void plan_fused(cupss_system s)
         // field); with an implicit operator setNotDynamic_k would divide the
         // kept value every step, which the general program reproduces
         if (F.terms.empty() && !F.imp.empty()) return;
+        if (F.noisy && s->g.sz > 1) return;
         stored[i] = 3;
     }
     // A constraint whose products read a dynamic field is "fresh" (kind 2):

@@ -159,8 +159,7 @@ int cupss_system_set_stream(cupss_system s, void *hip_stream);
  * or of derived and other such constraint fields) and coefficient fields
  * without terms; the fused 3D step takes one dynamic field with fields
  * linear in it and spectral-space noise.  Neither takes field callbacks
- * (either kind), noisy constraint fields or grids outside 16..4096 x
- * 8..4096 (x 8..1024). */
+ * (either kind) or grids outside 16..4096 x 8..4096 (x 8..1024). */
 int cupss_system_set_path(cupss_system s, int mode);
 /* Noise draw: 0 (default) = xi ~ N(0,1) per real-space cell, then R2C (the
  * reference's createNoise, src/field.cpp:300-316; the oracle reproduces it

@@ -31,6 +31,11 @@ state.
 * A constraint field without terms keeps its value (a coefficient field;
   setNotDynamic_k writes nothing): example 07's inhomogeneous diffusion
   without its callbacks, against the oracle.
+* Noisy constraint fields (setNotDynamic_k adds the noise times 1/sqrt(dt)
+  before the implicit division): a pure-noise field, a constraint of
+  dynamic-field products and a constraint solved for the next step, which
+  draws that step's noise; fused against the general program with the
+  same Philox counters, in both noise modes.
 * Parameter changes in the middle of a run (evolver::updateParameter,
   src/evolver.cpp:386-394) reach the second dynamic field's terms and
   implicit operator: fused against general, and a batch member against the
@@ -317,3 +322,37 @@ def test_parameter_update_two_dynamic_fields():
         a.step(10)
         for f in ia:
             assert np.array_equal(a.get_real(ia[f]), members[i][0].get_real(members[i][1][f])), (i, f)
+
+
+def spec_noisy_constraints(dims=(64, 64, 1), seed=54):
+    """dt phi + q^2 phi + q^4 phi = -q^2 mu + 0.2 iqx f + 0.3 c
+       mu = phi^3 + noise;   f = noise;   c (1 + q^2) = 0.5 mu + noise"""
+    n = dims[0] * dims[1] * dims[2]
+    x0 = np.random.default_rng(seed).uniform(-0.1, 0.1, n).astype(np.float32)
+    terms = [("phi", ["mu"], [(-1.0, 1)]), ("phi", ["f"], [(0.2, 0, 1)]), ("phi", ["c"], [(0.3,)]),
+             ("mu", ["phi", "phi", "phi"], [(1.0,)]), ("c", ["mu"], [(0.5,)])]
+    return {"dims": dims, "dt": 0.01,
+            "fields": [("phi", True), ("mu", False), ("f", False), ("c", False)],
+            "implicit": {"phi": [(1.0, 1), (-1.0, 2)], "c": [(1.0,), (1.0, 1)]}, "terms": terms,
+            "noise": {"mu": ((0.001,), seed + 1), "f": ((0.002, 1), seed + 2), "c": ((0.001,), seed + 3)},
+            "init": {"phi": x0}}
+
+
+@pytest.mark.parametrize("noise_mode", [0, 1])
+def test_noisy_constraints_fused_equal_general(noise_mode):
+    spec = spec_noisy_constraints()
+    runs = {}
+    for path in (0, 1):
+        g, ids = gpu(spec, path=path, noise_mode=noise_mode, profile=path == 0)
+        if path == 0:
+            assert g.fused_eligible
+        for n in (7, 1, 12):
+            g.step(n)
+        if path == 0:
+            assert_fused_stored_dynamic(g)
+        runs[path] = (g, ids)
+    for f in ("phi", "mu", "f", "c"):
+        a, b = runs[0][0].get_real(runs[0][1][f]), runs[1][0].get_real(runs[1][1][f])
+        e = rel_l2(a, b)
+        print("  noise mode %d, %-3s fused-vs-general %.3e" % (noise_mode, f, e), flush=True)
+        assert e <= 1e-5, f
