@@ -641,10 +641,12 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
                     }
                 }
             } else if (TG.keep) {
-                // no terms: setNotDynamic_k leaves the value as it is
+                // no terms: setNotDynamic_k leaves the value as it is, then
+                // divides it by the implicit operator's sum (if any)
                 const float2 *__restrict__ kin = a.cs[ci][parity];
 #pragma unroll
                 for (int e = 0; e < E; e++) v[e] = active ? gload(kin, col + t + e * T) : make_float2(0.f, 0.f);
+                constraint_solve<E>(a, TG, cb, v);
             } else {
                 if (XDYN && TG.noisy) stored_noise<N, E, T>(a, TG, cb, kx, step, active, col, L, t, tws, v);
                 constraint_solve<E>(a, TG, cb, v);

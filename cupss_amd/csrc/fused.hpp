@@ -11,7 +11,7 @@
 //             fields or constants ("fresh": solved at the start of a pass);
 //             then any other constraint field whose product
 //             terms read only derived or stored constraint fields (or that
-//             has no terms and no implicit operator: it keeps its value) -- kept as
+//             has no terms: it keeps its value) -- kept as
 //             state, updated every pass (the "phase A/B" of the next step,
 //             see fused.hip).  No product reads a fresh constraint.
 #pragma once
@@ -70,7 +70,8 @@ struct FusedTarget {
     int dyn;         // stored dynamic field: semi-implicit Euler step of its own value
     int fresh;       // stored constraint solved at the start of the pass from the dynamic
                      // fields before the step (its products read dynamic fields)
-    int keep;        // stored constraint without terms: keeps its value (a coefficient field)
+    int keep;        // stored constraint without terms: keeps its value (a coefficient field),
+                     // divided by its implicit operator's sum every step if it has one
     int noisy;       // stored field: its noise mode (NoiseMode), amplitude as UpdateArgs
     float noise_base;
     int noise_q2n, noise_invq;

@@ -890,10 +890,8 @@ static void plan_fused(cupss_system s)
                 continue;
             }
         }
-        // no terms and no implicit operator: it keeps its value (a coefficient
-        // field); with an implicit operator setNotDynamic_k would divide the
-        // kept value every step, which the general program reproduces
-        if (F.terms.empty() && !F.imp.empty()) return;
+        // no terms: it keeps its value (a coefficient field), divided by its
+        // implicit operator's sum every step if it has one (setNotDynamic_k)
         if (F.noisy && s->g.sz > 1) return;
         stored[i] = 3;
     }

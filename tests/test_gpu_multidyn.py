@@ -266,6 +266,24 @@ def spec_inhomogeneous_diffusion(n=128, seed=49):
             "implicit": {"v": [(-0.5, 1)]}, "terms": terms, "init": {"v": v.ravel(), "x": x.ravel()}}
 
 
+def test_termless_constraint_with_implicit_operator():
+    """A constraint without terms but with an implicit operator keeps its
+    value and divides it by the operator's sum every step (q != 0), as
+    setNotDynamic_k does: x (1 + 0.5 q^2) = (nothing), read by v's products."""
+    spec = spec_inhomogeneous_diffusion(64, seed=55)
+    spec["implicit"] = dict(spec["implicit"], x=[(1.0,), (0.5, 1)])
+    g, gid = gpu(spec, profile=True)
+    assert g.fused_eligible
+    o, oid = build_system(O.System, O.pres, dict(spec, precision="f32"))
+    g.step(30)
+    o.step(30)
+    for f in ("v", "x"):
+        err = rel_l2(g.get_real(gid[f]), o.get_real(oid[f]))
+        print("  %-2s gpu-vs-f32 %.3e" % (f, err), flush=True)
+        assert err <= 1e-4, f
+    assert rel_l2(g.get_real(gid["x"]), spec["init"]["x"]) > 1e-3  # x is smoothed every step
+
+
 def test_coefficient_field_keeps_its_value():
     spec = spec_inhomogeneous_diffusion()
     g, gid = gpu(spec, profile=True)
