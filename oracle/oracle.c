@@ -115,6 +115,16 @@ void orc_set_ref_cpu_dealias_bug(int on)
     orc_ref_cpu_dealias_bug_flag_f64 = on;
 }
 
+/* Planted mistakes, for the tests that show a check can fail (never set
+ * outside those tests):
+ *   ORC_PLANT_PHASES    advanceTime runs the dynamic phases (C, D) before
+ *                       the constraint phases (A, B)
+ *   ORC_PLANT_UNDEALIAS products read a factor's `real` array instead of
+ *                       its dealiased copy */
+static int orc_plant_flag;
+
+void orc_set_plant(int code) { orc_plant_flag = code; }
+
 /* ------------------------------------------------ float / double bodies */
 
 #define CAT2(a, b) a##b

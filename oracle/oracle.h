@@ -68,6 +68,17 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 float orc_normal(uint64_t seed, uint64_t step, uint32_t stream, uint32_t cell);
 void orc_normal_fill(uint64_t seed, uint64_t step, uint32_t stream, float *out, int n);
 
+/* Field callbacks, as the reference's field::callbackFourier / callback
+ * (src/field.cpp:48-58, 70-86): `data` is the field's full spectrum (after
+ * the update, before dealiasing) or its real array (after normalisation,
+ * then the dealiased copy too), interleaved (re, im) of N complex values in
+ * the oracle's precision; `user` is passed through. */
+typedef void (*orc_hook)(void *user, void *data, int sx, int sy, int sz);
+
+#define ORC_PLANT_PHASES 1
+#define ORC_PLANT_UNDEALIAS 2
+void orc_set_plant(int code);
+
 #define ORC_DECL(PFX, REAL)                                                       \
     typedef struct PFX##sys PFX##sys;                                             \
     PFX##sys *PFX##create(int sx, int sy, int sz, float dx, float dy, float dz,   \
@@ -79,6 +90,8 @@ void orc_normal_fill(uint64_t seed, uint64_t step, uint32_t stream, float *out, 
                       const orc_pres *pres);                                      \
     void PFX##set_noise(PFX##sys *s, int f, orc_pres amp, uint64_t seed);         \
     void PFX##set_real(PFX##sys *s, int f, const float *x);                       \
+    void PFX##set_hooks(PFX##sys *s, int f, orc_hook fourier, orc_hook real,      \
+                        void *user);                                              \
     void PFX##prepare(PFX##sys *s);                                               \
     void PFX##step(PFX##sys *s);                                                  \
     void PFX##steps(PFX##sys *s, int n);                                          \

@@ -21,6 +21,19 @@ class Pres(ctypes.Structure):
                 ("iqy", ctypes.c_int), ("iqz", ctypes.c_int), ("invq", ctypes.c_int)]
 
 
+# void hook(void *user, void *data, int sx, int sy, int sz)
+HOOK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int)
+
+PLANT_NONE, PLANT_PHASES, PLANT_UNDEALIAS = 0, 1, 2
+
+
+def set_plant(code):
+    """Plant a known mistake in the oracle (tests that show a check can
+    fail): PLANT_PHASES runs the dynamic phases before the constraint ones,
+    PLANT_UNDEALIAS makes products read undealiased factors."""
+    lib().orc_set_plant(int(code))
+
+
 def pres(pre=1.0, q2n=0, iqx=0, iqy=0, iqz=0, invq=0):
     return Pres(float(pre), int(q2n), int(iqx), int(iqy), int(iqz), int(invq))
 
@@ -66,6 +79,9 @@ def lib():
         L.orc_normal_fill.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
                                       ctypes.POINTER(ctypes.c_float), ctypes.c_int]
         L.orc_set_ref_cpu_dealias_bug.argtypes = [ctypes.c_int]
+        L.orc_set_plant.argtypes = [ctypes.c_int]
+        for pfx in ("orc_f32_", "orc_f64_"):
+            getattr(L, pfx + "set_hooks").argtypes = [ctypes.c_void_p, ctypes.c_int, HOOK, HOOK, ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -151,6 +167,35 @@ class System:
         x = np.ascontiguousarray(np.asarray(x, np.float32).ravel())
         assert x.size == self.n
         self._f("set_real")(self.h, f, _ptr(x, ctypes.c_float))
+
+    def set_hooks(self, f, fourier=None, real=None):
+        """Field callbacks (src/field.cpp:48-58, 70-86).  Each is a Python
+        function taking a complex128 view-copy of the N values (sz, sy, sx)
+        and returning the new values (or None to leave them): `fourier` sees
+        the updated full spectrum before dealiasing, `real` the normalised
+        real array (imaginary parts zero), then the dealiased copy."""
+        dims = self.dims
+        real_t = self.real
+
+        def wrap(fn):
+            if fn is None:
+                return HOOK()
+
+            def cb(_user, data, sx, sy, sz):
+                n = sx * sy * sz
+                buf = np.ctypeslib.as_array(ctypes.cast(data, ctypes.POINTER(self.ct)), shape=(2 * n,))
+                z = (buf[0::2] + 1j * buf[1::2]).reshape(dims[2], dims[1], dims[0])
+                out = fn(z)
+                if out is not None:
+                    out = np.asarray(out).reshape(-1)
+                    buf[0::2] = out.real.astype(real_t)
+                    buf[1::2] = out.imag.astype(real_t)
+            return HOOK(cb)
+
+        hooks = (wrap(fourier), wrap(real))
+        self._hooks = getattr(self, "_hooks", {})
+        self._hooks[f] = hooks  # keep the callbacks alive as long as the system
+        self._f("set_hooks")(self.h, f, hooks[0], hooks[1], None)
 
     def prepare(self):
         self._f("prepare")(self.h)

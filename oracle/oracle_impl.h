@@ -42,6 +42,9 @@ typedef struct {
     CPLX *noise_comp;
     REAL *precomp_implicit;
     REAL *precomp_noise;
+    orc_hook hook_fourier;
+    orc_hook hook_real;
+    void *hook_user;
 } FIELD;
 
 struct FN(sys) {
@@ -253,6 +256,13 @@ void FN(set_real)(FN(sys) *s, int f, const float *x)
     }
 }
 
+void FN(set_hooks)(FN(sys) *s, int f, orc_hook fourier, orc_hook real, void *user)
+{
+    s->f[f].hook_fourier = fourier;
+    s->f[f].hook_real = real;
+    s->f[f].hook_user = user;
+}
+
 static inline void FN(qvec)(const FN(sys) *s, int i, int j, int k, REAL *qx, REAL *qy, REAL *qz)
 {
     /* q_d(i) = (i < (s+1)/2 ? i : i - s) * stepq  (e.g. src/term_init.cpp:157-160) */
@@ -398,7 +408,10 @@ static void FN(term_update)(FN(sys) *s, TERM *t)
     } else {
         for (int c = 0; c < n; c++) {
             REAL r = 1;
-            for (int p = 0; p < t->nfac; p++) r *= s->f[t->fac[p]].real_dealiased[c].re;
+            for (int p = 0; p < t->nfac; p++) {
+                const FIELD *G = &s->f[t->fac[p]];
+                r *= (orc_plant_flag == ORC_PLANT_UNDEALIAS ? G->real : G->real_dealiased)[c].re;
+            }
             t->term_real[c].re = r;
             t->term_real[c].im = 0;
         }
@@ -489,6 +502,8 @@ static void FN(set_rhs)(FN(sys) *s, FIELD *F)
                     }
                 }
     }
+    /* Fourier-space callback on the updated spectrum (field.cpp:48-58) */
+    if (F->hook_fourier) F->hook_fourier(F->hook_user, F->comp, sx, sy, sz);
     /* dealias (field.cpp:59-61 -> dealias_k, field_kernels.cu:229-256) */
     if (F->needsaliasing) {
         int p1 = F->aliasing_order + 1;
@@ -522,6 +537,11 @@ static void FN(set_rhs)(FN(sys) *s, FIELD *F)
             F->real_dealiased[c].im = 0;
         }
     }
+    /* real-space callback on real, then on real_dealiased (field.cpp:70-86) */
+    if (F->hook_real) {
+        F->hook_real(F->hook_user, F->real, sx, sy, sz);
+        if (F->needsaliasing) F->hook_real(F->hook_user, F->real_dealiased, sx, sy, sz);
+    }
     /* toComp (field.cpp:261-273); the forward transform of real_dealiased
      * (:266) is never read and is skipped. */
     FN(fftnd)(F->real, F->comp, sx, sy, sz, -1);
@@ -530,6 +550,18 @@ static void FN(set_rhs)(FN(sys) *s, FIELD *F)
 /* evolver::advanceTime, src/evolver.cpp:199-226 (output omitted). */
 void FN(step)(FN(sys) *s)
 {
+    if (orc_plant_flag == ORC_PLANT_PHASES) {
+        for (int i = 0; i < s->nfields; i++)
+            if (s->f[i].dynamic) FN(update_terms)(s, &s->f[i]);
+        for (int i = 0; i < s->nfields; i++)
+            if (s->f[i].dynamic) FN(set_rhs)(s, &s->f[i]);
+        for (int i = 0; i < s->nfields; i++)
+            if (!s->f[i].dynamic) FN(update_terms)(s, &s->f[i]);
+        for (int i = 0; i < s->nfields; i++)
+            if (!s->f[i].dynamic) FN(set_rhs)(s, &s->f[i]);
+        s->step += 1;
+        return;
+    }
     for (int i = 0; i < s->nfields; i++)
         if (!s->f[i].dynamic) FN(update_terms)(s, &s->f[i]);
     for (int i = 0; i < s->nfields; i++)

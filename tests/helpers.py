@@ -59,6 +59,20 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
+def lowpass(cut):
+    """Oracle Fourier callback: zero every mode with |n_x| > cut or
+    |n_y| > cut (signed mode numbers, n > N/2 counted as n - N)."""
+    def fn(z):
+        _, sy, sx = z.shape
+        nx = np.arange(sx)
+        nx = np.where(nx > sx // 2, nx - sx, nx)
+        ny = np.arange(sy)
+        ny = np.where(ny > sy // 2, ny - sy, ny)
+        keep = (np.abs(nx)[None, None, :] <= cut) & (np.abs(ny)[None, :, None] <= cut)
+        return np.where(keep, z, 0)
+    return fn
+
+
 def f32(x):
     """The float32 value of a parameter, as the reference stores it."""
     return float(np.float32(x))
