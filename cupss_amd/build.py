@@ -18,9 +18,10 @@ LIB = os.path.join(HERE, "libcupss.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CUPSS_ARCH", "gfx950")
 
-HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
-             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-             "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
+# device code never gets packed FP32 instructions (v_pk_*_f32)
+NO_PACKED_FP32 = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function"] + \
+    NO_PACKED_FP32 + ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 
 
 def sources():
@@ -64,7 +65,8 @@ def _build_tool(src):
     os.makedirs(bindir, exist_ok=True)
     exe = os.path.join(bindir, os.path.splitext(os.path.basename(src))[0])
     rpath = os.path.relpath(HERE, bindir)
-    cmd = [HIPCC, "--offload-arch=" + ARCH, "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), src,
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-O2", "-std=c++17"] + NO_PACKED_FP32 + \
+        ["-I" + os.path.join(ROOT, "include"), src,
            "-o", exe, "-L" + HERE, "-lcupss", "-Wl,-rpath,$ORIGIN/" + rpath]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -22,6 +22,8 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 REF = "/root/reference/examples"
 OUT = os.path.join(HERE, "bin", "ref_examples")
 LIBDIR = os.path.join(ROOT, "cupss_amd")
+sys.path.insert(0, ROOT)
+from cupss_amd.build import NO_PACKED_FP32  # noqa: E402
 
 # (source under examples/, binary name)
 PROGRAMS = [("01_diffusion/01_diffusion.cpp", "01_diffusion"),
@@ -47,8 +49,8 @@ def build(verbose=False):
             with open(dst, "w") as f:
                 f.write(src)
             exe = os.path.join(OUT, name)
-            cmd = ["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-O2", "-std=c++17",
-                   "-I" + os.path.join(ROOT, "include")]
+            cmd = ["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-O2", "-std=c++17"] + NO_PACKED_FP32 + \
+                ["-I" + os.path.join(ROOT, "include")]
             if rel.endswith(".cu"):
                 cmd += ["-x", "hip"]
             cmd += [dst, "-o", exe, "-L" + LIBDIR, "-lcupss",

@@ -9,7 +9,11 @@
 // its field in Fourier space.  Each pair of runs must agree: bit for bit
 // for the real-space callbacks (same transforms, same values), within
 // 1e-6 for the Fourier ones (the float2 path projects the full spectrum
-// back onto the half spectrum).
+// back onto the half spectrum).  Both low-pass runs start from the same
+// initial field, drawn once from a fixed-seed generator and written through
+// real_array; it and the device run's final field are left in the working
+// directory (lowpass_ic.f32, lowpass_phi.f32: raw float32, x fastest) so
+// the caller can check the filtered run against the oracle.
 // Usage: hooks_driver   (exit status = number of failed checks)
 #include <hip/hip_runtime.h>
 
@@ -17,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <random>
 #include <vector>
 
 #include "../../include/cupss.h"
@@ -138,8 +143,30 @@ std::vector<float> mirrored(bool device, std::vector<float> *xout)
     return real_of(sys, "v");
 }
 
+// i.i.d. uniform in [-0.1, 0.1] from a fixed seed (the library's
+// initializeUniformNoise reseeds rand() from the clock, as the reference's
+// does, so it cannot give two runs the same start)
+std::vector<float> lowpass_ic()
+{
+    std::mt19937 gen(7);
+    std::uniform_real_distribution<float> u(-0.1f, 0.1f);
+    std::vector<float> x(4 * N * N);
+    for (float &v : x) v = u(gen);
+    return x;
+}
+
+void save(const char *path, const std::vector<float> &x)
+{
+    FILE *fh = std::fopen(path, "wb");
+    if (!fh || std::fwrite(x.data(), sizeof(float), x.size(), fh) != x.size()) {
+        std::printf("cannot write %s\n", path);
+        std::exit(99);
+    }
+    std::fclose(fh);
+}
+
 // Cahn-Hilliard whose field is low-pass filtered after every update
-std::vector<float> filtered(bool device)
+std::vector<float> filtered(bool device, const std::vector<float> &ic)
 {
     evolver sys(RUN_GPU, 2 * N, 2 * N, 1.0f, 1.0f, 0.01f, STEPS + 1);
     sys.createField("phi", true);
@@ -147,9 +174,8 @@ std::vector<float> filtered(bool device)
     sys.addParameter("b", 1.0f);
     sys.addParameter("k", 1.0f);
     sys.addEquation("dt phi + (a*q^2 + k*q^4)*phi = -b*q^2*phi^3");
-    std::srand(7);  // the constructor seeds rand() with the time, as the reference's does
-    sys.initializeUniformNoise("phi", 0.1f);
     field *f = sys.fieldsMap["phi"];
+    for (size_t i = 0; i < ic.size(); i++) f->real_array[i] = make_float2(ic[i], 0.0f);
     if (device) {
         f->hasCBFourierDevice = true;
         f->callbackFourierDevice = lowpass_half;
@@ -193,6 +219,11 @@ int main()
         for (int i = N + 1; i < 2 * N; i++) asym = std::fmax(asym, std::fabs(vb[j * 2 * N + i] - vb[j * 2 * N + 2 * N - i]));
     std::printf("%-34s %.3e\n", "mirror residual of v", asym);
     if (asym > 1e-5) bad++;
-    bad += compare("Fourier low-pass callbacks, phi", filtered(false), filtered(true), 1e-6);
+    const std::vector<float> ic = lowpass_ic();
+    const std::vector<float> pa = filtered(false, ic);
+    const std::vector<float> pb = filtered(true, ic);
+    save("lowpass_ic.f32", ic);
+    save("lowpass_phi.f32", pb);
+    bad += compare("Fourier low-pass callbacks, phi", pa, pb, 1e-6);
     return bad;
 }

@@ -19,7 +19,7 @@ import pytest
 
 from oracle import oracle as O
 from cupss_amd.systems import spec_gray_scott
-from tests.helpers import build_system, rel_l2, spec_cahn_hilliard, spec_model_h
+from tests.helpers import build_system, lowpass, rel_l2, spec_cahn_hilliard, spec_model_h
 
 pytestmark = pytest.mark.gpu
 
@@ -37,6 +37,25 @@ def run(args, cwd, timeout=240):
 def test_device_callbacks_match_float2_callbacks(tmp_path):
     r = run([os.path.join(BIN, "hooks_driver")], str(tmp_path))
     assert r.returncode == 0, r.stdout
+    # the low-pass filtered Cahn-Hilliard run (device-side Fourier callback)
+    # against the oracle with the same filter applied after each update, as
+    # src/field.cpp:48-58 does: 64^2, dt 0.01, a=-1, b=1, k=1, 200 steps,
+    # keep |n_x|, |n_y| <= 10
+    n, steps, cut = 64, 200, 10
+    ic = np.fromfile(str(tmp_path / "lowpass_ic.f32"), np.float32)
+    got = np.fromfile(str(tmp_path / "lowpass_phi.f32"), np.float32)
+    s = O.System(n, n, 1, 1.0, 1.0, 1.0, 0.01, "f32")
+    phi = s.add_field(True)
+    s.add_implicit(phi, O.pres(1.0, q2n=1))
+    s.add_implicit(phi, O.pres(-1.0, q2n=2))
+    s.add_term(phi, [phi] * 3, [O.pres(-1.0, q2n=1)])
+    s.set_real(phi, ic)
+    s.set_hooks(phi, fourier=lowpass(cut))
+    s.prepare()
+    s.step(steps)
+    err = rel_l2(got, s.get_real(phi))
+    print("low-pass Cahn-Hilliard vs oracle: rel L2 %.3e" % err)
+    assert err < 1e-4
 
 
 def test_reference_cases(tmp_path):
