@@ -10,6 +10,7 @@
 #include <ctime>
 #include <iomanip>
 #include <iostream>
+#include <sstream>
 
 #include "../../../include/cupss.h"
 #include "../../../include/cupss_hip.h"
@@ -323,70 +324,60 @@ void evolver::writeOut()
 void evolver::setVerbose() { verbose = true; }
 void evolver::unsetVerbose() { verbose = false; }
 
+// A summary of the system as parsed: the grid, then one block per field
+// with its kind, its dealiasing and its equation written back in the
+// parser's own grammar (operator monomials as c*q^2n*iqx^a*iqy^b*iqz^c*1/q^k,
+// implicit operators on the left, each term as [operator sum](factors)).
 void evolver::printInformation()
 {
-    std::cout << std::fixed;
-    std::cout << std::setprecision(3);
-    std::cout << "Information on this evolver:" << std::endl;
-    std::cout << dimension << "-dimensional system of size " << sx << "x" << sy << "x" << sz << std::endl;
-    std::cout << "Physical size " << (float)sx * dx << "x" << (float)sy * dy << "x" << (float)sz * dz
-              << " with cells of size " << dx << "x" << dy << "x" << dz << std::endl;
-    std::cout << "There are " << fields.size() << " fields." << std::endl;
+    auto monomial = [](const pres &p) {
+        std::ostringstream os;
+        os << p.preFactor;
+        if (p.q2n) os << "*q^" << 2 * p.q2n;
+        if (p.iqx) os << "*iqx^" << p.iqx;
+        if (p.iqy) os << "*iqy^" << p.iqy;
+        if (p.iqz) os << "*iqz^" << p.iqz;
+        if (p.invq) os << "*1/q^" << p.invq;
+        return os.str();
+    };
+    auto sum = [&](const std::vector<pres> &ps) {
+        std::string out;
+        for (size_t i = 0; i < ps.size(); i++) out += (i ? " + " : "") + monomial(ps[i]);
+        return "[" + out + "]";
+    };
+    std::ostringstream os;
+    os << "evolver: " << dimension << "D grid " << sx << " x " << sy << " x " << sz << ", cells " << dx << " x " << dy
+       << " x " << dz << ", dt " << dt << ", step " << currentTimeStep << ", " << (with_cuda ? "GPU" : "CPU mode")
+       << ", " << fields.size() << " field(s)\n";
     for (size_t i = 0; i < fields.size(); i++) {
-        field *f = fields[i];
-        std::cout << "Field " << i << ": " << f->name;
-        std::cout << (f->dynamic ? " is dynamic." : " is not dynamic");
-        std::cout << " and has " << f->terms.size() << " explicit terms";
-        std::cout << " and " << f->implicit.size() << " implicit terms.";
-        std::cout << " Runs on GPU: " << f->isCUDA;
+        const field *f = fields[i];
+        os << "  [" << i << "] " << f->name << ": " << (f->dynamic ? "dynamic" : "constraint") << ", "
+           << f->terms.size() << " term(s), " << f->implicit.size() << " implicit operator(s), ";
         if (f->needsaliasing)
-            std::cout << ". Will be dealiased for a nonlinearity of order " << f->aliasing_order;
+            os << "dealiased for products of order " << f->aliasing_order << " (keeps |n| <= N/"
+               << f->aliasing_order + 1 << ")";
         else
-            std::cout << ". Will not be dealised.";
-        std::cout << std::endl << "\t";
-        if (f->dynamic) std::cout << "(d/dt)";
-        std::cout << f->name;
-        if (f->dynamic) std::cout << " = ";
-        auto op = [](const pres &p) {
-            std::string s;
-            if (p.iqx != 0) s += "(iqx)^(" + std::to_string(p.iqx) + ")";
-            if (p.iqy != 0) s += "(iqy)^(" + std::to_string(p.iqy) + ")";
-            if (p.iqz != 0) s += "(iqz)^(" + std::to_string(p.iqz) + ")";
-            if (p.q2n != 0) s += "(q)^(" + std::to_string(2 * p.q2n) + ")";
-            if (p.invq != 0) s += "(1/|q|)^(" + std::to_string(p.invq) + ")";
-            return s;
-        };
+            os << "not dealiased";
+        os << "\n      " << (f->dynamic ? "dt " : "") << f->name;
         if (!f->implicit.empty()) {
-            std::string line = "[";
-            for (const pres &p : f->implicit) {
-                if (p.preFactor > 0.0f) line += "+";
-                line += std::to_string(p.preFactor) + op(p);
-            }
-            line += "]";
-            if (f->dynamic) line += f->name;
-            std::cout << line;
+            // the parser stores a dynamic field's implicit operators negated
+            std::vector<pres> lhs = f->implicit;
+            if (f->dynamic)
+                for (pres &p : lhs) p.preFactor = -p.preFactor;
+            os << (f->dynamic ? " + " : " * ") << sum(lhs) << (f->dynamic ? " " + f->name : "");
         }
-        if (!f->dynamic) std::cout << " = ";
+        os << " =";
+        if (f->terms.empty() && !f->isNoisy) os << " (unchanged)";
         for (size_t j = 0; j < f->terms.size(); j++) {
-            std::string line = j != 0 ? " + [" : " [";
-            const std::vector<pres> &ps = f->terms[j]->prefactors_h;
-            for (size_t p = 0; p < ps.size(); p++) {
-                line += " + (" + std::to_string(ps[p].preFactor) + ")" + op(ps[p]);
-                if (p != ps.size() - 1) line += " + ";
-            }
-            line += "] (";
-            for (field *g : f->terms[j]->product) line += " " + g->name;
-            line += " )";
-            std::cout << line;
+            const term *t = f->terms[j];
+            os << (j ? " + " : " ") << sum(t->prefactors_h) << "(";
+            for (size_t c = 0; c < t->product.size(); c++) os << (c ? "*" : "") << t->product[c]->name;
+            os << (t->product.empty() ? "1" : "") << ")";
         }
-        if (f->isNoisy) {
-            std::cout << "+ sqrt[2*" << f->noise_amplitude.preFactor;
-            if (f->noise_amplitude.q2n != 0) std::cout << "*q^" << f->noise_amplitude.q2n * 2;
-            if (f->noise_amplitude.invq != 0) std::cout << "*1/|q|^" << f->noise_amplitude.q2n;
-            std::cout << "] x noise";
-        }
-        std::cout << std::endl << std::endl;
+        if (f->isNoisy) os << (f->terms.empty() ? " " : " + ") << "noise of variance " << monomial(f->noise_amplitude);
+        os << "\n";
     }
+    std::cout << os.str() << std::flush;
 }
 
 int evolver::createTerm(const std::string &_field, const std::vector<pres> &_prefactors,
