@@ -96,6 +96,7 @@ _SIGS = {
     "cupss_field_spectrum_device": (_I, [_VP, _I, ctypes.POINTER(_VP)]),
     "cupss_r2c": (_I, [_I, _I, _I, _VP, _VP, _VP]),
     "cupss_c2r": (_I, [_I, _I, _I, _VP, _I, _F, _VP, _VP, _VP]),
+    "cupss_c2r_noalloc": (_I, [_I, _I, _I, _VP, _I, _F, _VP, _VP]),
     "cupss_normals": (_I, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, _VP, ctypes.c_long, _VP]),
     "cupss_device_alloc": (_I, [ctypes.c_size_t, ctypes.POINTER(_VP)]),
     "cupss_device_free": (_I, [_VP]),

@@ -76,6 +76,8 @@ struct MaskArgs {
 // Twiddle table (device) for the current device, created on first use:
 // exp(-2 pi i j / TW_N), j < TW_N.
 const float2 *twiddles(hipStream_t s);
+// the TW_N-entry table twiddles() uploads, computed on the host
+void twiddle_table(float2 *host_out);
 // All power-of-two tables: table M (exp(-2 pi i j / M), j < M) at offset M,
 // M = 1, 2, 4, ..., TW_N.
 const float2 *twiddle_sets(hipStream_t s);
@@ -95,6 +97,9 @@ hipError_t r2c(const Geom &g, const R2CSource &src, float2 *out, hipStream_t s);
 // real out[n] <- scale * C2R(mask_p(in)); scratch[nh] is clobbered (may not alias in)
 hipError_t c2r(const Geom &g, const float2 *in, int mask_p, float scale, float *out, float2 *scratch,
                hipStream_t s);
+// the same with the caller's twiddle table (TW_N entries, twiddle_table())
+hipError_t c2r_tw(const Geom &g, const float2 *in, int mask_p, float scale, float *out, float2 *scratch,
+                  hipStream_t s, const float2 *tw);
 // the update of one field, then the Hermitian projection of its kx = 0 and
 // kx = sx/2 planes (hermitian_planes)
 hipError_t update(const UpdateArgs &a, hipStream_t s);

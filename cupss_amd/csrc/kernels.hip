@@ -31,6 +31,14 @@ static std::mutex g_tw_mutex;
 static std::vector<float2 *> g_tw;    // per device
 static std::vector<float2 *> g_tws;   // per device
 
+void twiddle_table(float2 *h)
+{
+    for (int j = 0; j < TW_N; j++) {
+        double a = -2.0 * M_PI * (double)j / (double)TW_N;
+        h[j] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+}
+
 const float2 *twiddles(hipStream_t s)
 {
     int dev = 0;
@@ -39,10 +47,7 @@ const float2 *twiddles(hipStream_t s)
     if ((int)g_tw.size() <= dev) g_tw.resize(dev + 1, nullptr);
     if (!g_tw[dev]) {
         std::vector<float2> h(TW_N);
-        for (int j = 0; j < TW_N; j++) {
-            double a = -2.0 * M_PI * (double)j / (double)TW_N;
-            h[j] = make_float2((float)std::cos(a), (float)std::sin(a));
-        }
+        twiddle_table(h.data());
         float2 *d = nullptr;
         if (hipMalloc(&d, sizeof(float2) * TW_N) != hipSuccess) return nullptr;
         if (hipMemcpy(d, h.data(), sizeof(float2) * TW_N, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
@@ -403,6 +408,12 @@ hipError_t c2r(const Geom &g, const float2 *in, int mask_p, float scale, float *
 {
     const float2 *tw = twiddles(s);
     if (!tw) return hipErrorOutOfMemory;
+    return c2r_tw(g, in, mask_p, scale, out, scratch, s, tw);
+}
+
+hipError_t c2r_tw(const Geom &g, const float2 *in, int mask_p, float scale, float *out, float2 *scratch,
+                  hipStream_t s, const float2 *tw)
+{
     hipError_t e;
     const long plane = (long)g.nc1 * g.sy;
     const float2 *cur = in;

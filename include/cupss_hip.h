@@ -265,6 +265,16 @@ int cupss_field_spectrum_device(cupss_system s, int field, void **dev_ptr);
 int cupss_r2c(int sx, int sy, int sz, const float *in, void *out, void *hip_stream);
 int cupss_c2r(int sx, int sy, int sz, const void *in, int mask_p, float scale, float *out, void *scratch,
               void *hip_stream);
+/* cupss_c2r without its workspace argument and without any device
+ * allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc).  The workspace is a static per-device array of
+ * 2^21 float2 (16 MiB, held until the process ends): half spectra of more
+ * bins ((sx/2+1)*sy*sz > 2^21) are rejected with CUPSS_ERR_INVALID.  The
+ * first call on a device sets up (a synchronous 64 KiB twiddle upload);
+ * calls that overlap on one device share the array and race. */
+int cupss_c2r_noalloc(int sx, int sy, int sz, const void *in, int mask_p, float scale, float *out,
+                      void *hip_stream);
 /* Philox normals of a stream (seed, step, stream) for cells [0, n). */
 int cupss_normals(uint64_t seed, uint64_t step, uint32_t stream, float *out, long n, void *hip_stream);
 
