@@ -191,12 +191,36 @@ __device__ __forceinline__ void load_twiddles(float2 *__restrict__ dst, const fl
     for (int i = threadIdx.x; i < M; i += blockDim.x) dst[i] = sets[M + i];
 }
 
-// One Stockham stage of radix R with span NS, then recurse.
-template <int N, int E, int DIR, int NS, int TWS = TW_N>
+// Synchronisation of one line's LDS exchanges.  WL (wave-local): the line's
+// lanes lie inside one wavefront and its LDS region is its own.  A wave's
+// LDS accesses are performed in program order, so its lanes then exchange
+// values through LDS with no workgroup barrier: the fences (wave scope,
+// which emit no instruction) only keep the compiler from moving accesses
+// across the exchange.  Otherwise a workgroup barrier.
+template <bool WL>
+__device__ __forceinline__ void line_sync()
+{
+    if constexpr (WL) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
+// One Stockham stage of radix R with span NS, then recurse.  OUT_LDS: the
+// last stage also leaves the transform in LDS (natural order); without it
+// the last stage's butterfly outputs go straight back to the registers --
+// in the strided distribution they already sit in the slots their inputs
+// came from (output t + T*(b + r*E/R) of butterfly b) -- and the final LDS
+// round trip is skipped.
+template <int N, int E, int DIR, int NS, int TWS = TW_N, bool OUT_LDS = true, bool WL = false>
 struct Stockham {
     static constexpr int T = N / E;
     static constexpr int REM = N / NS;
     static constexpr int R = REM >= E ? E : REM;
+    static constexpr bool LAST = NS * R == N;
     __device__ __forceinline__ static void run(float2 (&v)[E], float2 *lds, int t, const float2 *__restrict__ tw)
     {
         if constexpr (NS < N) {
@@ -212,36 +236,46 @@ struct Stockham {
                     for (int r = 1; r < R; r++) a[r] = cmul(a[r], tw_lookup<DIR, TWS>(tw, r * k, NS * R));
                 }
                 Dft<R, DIR>::run(a);
-                // lpad(base + r*NS) = lpad(base) + lpad(r*NS): base = (j-k)*R + k
-                // with (j-k)*R a multiple of NS*R, so the R outputs either stay
-                // inside one 32-entry block (NS*R <= 32) or step across blocks
-                // exactly as r*NS does.  One address register, constant offsets.
-                const int wb = lpad((j - k) * R + k);
+                if constexpr (LAST && !OUT_LDS) {
 #pragma unroll
-                for (int r = 0; r < R; r++) lds[wb + lpad(r * NS)] = a[r];
+                    for (int r = 0; r < R; r++) v[b + r * (E / R)] = a[r];
+                } else {
+                    // lpad(base + r*NS) = lpad(base) + lpad(r*NS): base = (j-k)*R + k
+                    // with (j-k)*R a multiple of NS*R, so the R outputs either stay
+                    // inside one 32-entry block (NS*R <= 32) or step across blocks
+                    // exactly as r*NS does.  One address register, constant offsets.
+                    const int wb = lpad((j - k) * R + k);
+#pragma unroll
+                    for (int r = 0; r < R; r++) lds[wb + lpad(r * NS)] = a[r];
+                }
             }
-            __syncthreads();
-            // lpad(t + e*T) = lpad(t) + lpad(e*T) for t < T, T a power of two
-            const int rb = lpad(t);
+            if constexpr (!(LAST && !OUT_LDS)) {
+                line_sync<WL>();
+                // lpad(t + e*T) = lpad(t) + lpad(e*T) for t < T, T a power of two
+                const int rb = lpad(t);
 #pragma unroll
-            for (int e = 0; e < E; e++) v[e] = lds[rb + lpad(e * T)];
-            if constexpr (NS * R < N) __syncthreads();
-            Stockham<N, E, DIR, NS * R, TWS>::run(v, lds, t, tw);
+                for (int e = 0; e < E; e++) v[e] = lds[rb + lpad(e * T)];
+                if constexpr (!LAST) line_sync<WL>();
+            }
+            Stockham<N, E, DIR, NS * R, TWS, OUT_LDS, WL>::run(v, lds, t, tw);
         }
     }
 };
 
 // Full transform of one line.  On entry v[e] = x[t + e*T]; on exit
-// v[e] = X[t + e*T] and lds[lpad(k)] = X[k] for all k (after a barrier).
-// All threads of the workgroup must call it (it contains barriers).
-template <int N, int E, int DIR, int TWS = TW_N>
+// v[e] = X[t + e*T] and, with OUT_LDS, lds[lpad(k)] = X[k] for all k (after
+// a line_sync).  With WL false all threads of the workgroup must call it
+// (it contains barriers); WL as for line_sync.
+template <int N, int E, int DIR, int TWS = TW_N, bool OUT_LDS = true, bool WL = false>
 __device__ __forceinline__ void fft_line(float2 (&v)[E], float2 *lds, int t, const float2 *__restrict__ tw)
 {
     if constexpr (N == 1) {
-        lds[0] = v[0];
-        __syncthreads();
+        if constexpr (OUT_LDS) {
+            lds[0] = v[0];
+            line_sync<WL>();
+        }
     } else {
-        Stockham<N, E, DIR, 1, TWS>::run(v, lds, t, tw);
+        Stockham<N, E, DIR, 1, TWS, OUT_LDS, WL>::run(v, lds, t, tw);
     }
 }
 

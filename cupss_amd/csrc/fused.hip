@@ -129,7 +129,7 @@ __device__ __forceinline__ void c2r_prep(const float2 (&xa)[E], const float2 (&y
 template <int NC, int E, int T>
 __device__ __forceinline__ void r2c_tile(float2 (&v)[E], float2 *L, int t, const float2 *tws)
 {
-    fft_line<NC, E, -1, 2 * NC>(v, L, t, tws);
+    fft_line<NC, E, -1, 2 * NC, true, (T <= 64)>(v, L, t, tws);
     float2 o[E];
 #pragma unroll
     for (int e = 0; e < E; e++) {
@@ -141,7 +141,7 @@ __device__ __forceinline__ void r2c_tile(float2 (&v)[E], float2 *L, int t, const
         o[e] = k == 0 ? make_float2(xe.x + xo.x, 0.0f) : cadd(xe, cmul(tw_lookup<-1, 2 * NC>(tws, k, 2 * NC), xo));
     }
     const float2 z0 = L[lpad(0)];
-    __syncthreads();  // every read of the transform's LDS line is done
+    line_sync<(T <= 64)>();  // every read of the transform's LDS line is done
 #pragma unroll
     for (int e = 0; e < E; e++) L[t + e * T] = o[e];
     if (t == 0) L[NC] = make_float2(z0.x - z0.y, 0.0f);
@@ -183,7 +183,7 @@ __global__ __launch_bounds__((FRowCfg<NC, LPB, MAXR>::THREADS)) void k_frows(con
             }
             __syncthreads();  // the tile becomes the transform's LDS
             c2r_prep<NC, E, T>(xa, ya, t, v, tws);
-            fft_line<NC, E, +1, 2 * NC>(v, L, t, tws);
+            fft_line<NC, E, +1, 2 * NC, false, (T <= 64)>(v, L, t, tws);
 #pragma unroll
             for (int e = 0; e < E; e++) r[f][e] = cscale(v[e], a.scale);
             __syncthreads();
@@ -267,7 +267,7 @@ __global__ __launch_bounds__((FRowCfg<NC, LPB, MAXR>::THREADS)) void k_frows(con
 // Sum of the terms of one target field over the E bins of a lane: product
 // groups (FFT_y of X_g times the group multiplier) and single-field or
 // constant terms.  xv holds X of the first group on entry when `pre`.
-template <int N, int E, int T>
+template <int N, int E, int T, bool SKIP>
 __device__ __forceinline__ void target_sum(const FusedInst &a, const FusedTarget &TG, const ColBins<E> &cb,
                                            const float2 (&old)[E], const float2 (&nw)[E], float2 (&xv)[E],
                                            bool pre, int parity, bool active, long col, float2 *L, int t,
@@ -281,12 +281,12 @@ __device__ __forceinline__ void target_sum(const FusedInst &a, const FusedTarget
 #pragma unroll
             for (int e = 0; e < E; e++) xv[e] = active ? gload(a.X[gi], col + t + e * T) : make_float2(0.f, 0.f);
         }
-        fft_line<N, E, -1, N>(xv, L, t, tws);
+        fft_line<N, E, -1, N, !SKIP, (T <= 64)>(xv, L, t, tws);
         const TermRefDev &GM = a.gmul[gi];
         multiplier_col<E>(a.pool, GM.pres0, GM.npres, GM.mult_i, cb, m);
 #pragma unroll
         for (int e = 0; e < E; e++) acc[e] = cadd(acc[e], apply_mult(m[e], GM.mult_i, xv[e]));
-        __syncthreads();
+        line_sync<(T <= 64)>();
     }
     for (int ti = 0; ti < TG.nterms; ti++) {
         const FusedTerm &TR = TG.term[ti];
@@ -326,7 +326,7 @@ __device__ __forceinline__ void target_sum(const FusedInst &a, const FusedTarget
 }
 
 // Z_f = IFFT_y(mask_f * val) for the columns the mask keeps
-template <int N, int E, int T>
+template <int N, int E, int T, bool SKIP>
 __device__ __forceinline__ void z_out(const FusedInst &a, int f, const Geom &g, const ColBins<E> &cb,
                                       const float2 (&val)[E], bool zero, int line, bool active, long col, float2 *L,
                                       int t, const float2 *tws)
@@ -336,12 +336,12 @@ __device__ __forceinline__ void z_out(const FusedInst &a, int f, const Geom &g, 
 #pragma unroll
     for (int e = 0; e < E; e++)
         v[e] = (zero || !keep_mode2(g, a.kmax[f], a.kymax[f], cb.kx, t + e * T)) ? make_float2(0.f, 0.f) : val[e];
-    fft_line<N, E, +1, N>(v, L, t, tws);
+    fft_line<N, E, +1, N, !SKIP, (T <= 64)>(v, L, t, tws);
     if (active && cb.kx <= a.kmax[f]) {
 #pragma unroll
         for (int e = 0; e < E; e++) gstore(a.Z[f], col + t + e * T, v[e]);
     }
-    __syncthreads();
+    line_sync<(T <= 64)>();
 }
 
 // sum over the pool prefactors [p0, p0 + n) of c q^(2 q2n) |q|^-invq per bin
@@ -375,7 +375,7 @@ __device__ __forceinline__ void pool_sum_col(const PresDev *pool, int p0, int n,
 // noise of a stored field over the E bins of a lane, times its amplitude
 // (times 1/sqrt(dt) for a constraint): real-space rows transformed along y,
 // or drawn in spectral space, at the pass's step + step_off
-template <int N, int E, int T>
+template <int N, int E, int T, bool SKIP>
 __device__ __forceinline__ void stored_noise(const FusedInst &a, const FusedTarget &TG, const ColBins<E> &cb, int kx,
                                              uint64_t step, bool active, long col, float2 *L, int t,
                                              const float2 *tws, float2 (&v)[E])
@@ -386,8 +386,8 @@ __device__ __forceinline__ void stored_noise(const FusedInst &a, const FusedTarg
     if (TG.noisy == NOISE_REAL) {
 #pragma unroll
         for (int e = 0; e < E; e++) ue[e] = active ? gload(TG.eta, col + t + e * T) : make_float2(0.f, 0.f);
-        fft_line<N, E, -1, N>(ue, L, t, tws);
-        __syncthreads();
+        fft_line<N, E, -1, N, !SKIP, (T <= 64)>(ue, L, t, tws);
+        line_sync<(T <= 64)>();
     } else {
         const uint64_t s = step + (uint64_t)TG.step_off;
 #pragma unroll
@@ -445,7 +445,7 @@ __device__ __forceinline__ void pool_implicit_col(const PresDev *pool, int p0, i
 
 // Z_f = IFFT_y(mask_f * L_f u) of a real field that is a dynamic field u or
 // derived from it
-template <int N, int E, int T>
+template <int N, int E, int T, bool SKIP>
 __device__ __forceinline__ void lz_out(const FusedInst &a, int f, const Geom &g, const ColBins<E> &cb,
                                        const float2 (&u)[E], bool zero, int line, bool active, long col, float2 *L,
                                        int t, const float2 *tws)
@@ -461,7 +461,7 @@ __device__ __forceinline__ void lz_out(const FusedInst &a, int f, const Geom &g,
 #pragma unroll
         for (int e = 0; e < E; e++) v[e] = u[e];
     }
-    z_out<N, E, T>(a, f, g, cb, v, zero, line, active, col, L, t, tws);
+    z_out<N, E, T, SKIP>(a, f, g, cb, v, zero, line, active, col, L, t, tws);
 }
 
 // Hermitian projection of the columns kx = 0 and kx = sx/2 (2D): bin ky
@@ -478,7 +478,7 @@ __device__ __forceinline__ void hermitian_col(float2 (&v)[E], float2 *L, int t, 
 {
 #pragma unroll
     for (int e = 0; e < E; e++) L[lpad(t + e * T)] = v[e];
-    __syncthreads();
+    line_sync<(T <= 64)>();
     if (edge) {
 #pragma unroll
         for (int e = 0; e < E; e++) {
@@ -486,7 +486,7 @@ __device__ __forceinline__ void hermitian_col(float2 (&v)[E], float2 *L, int t, 
             v[e] = make_float2(0.5f * (v[e].x + p.x), 0.5f * (v[e].y - p.y));
         }
     }
-    __syncthreads();
+    line_sync<(T <= 64)>();
 }
 
 template <int N, int TWD>
@@ -560,8 +560,8 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
                 const FusedTarget &TG = a.tgt[1 + ci];
                 if (!TG.fresh) continue;
                 float2 v[E];
-                target_sum<N, E, T>(a, TG, cb, cur, cur, xv, false, parity, active, col, L, t, tws, v);
-                if (TG.noisy) stored_noise<N, E, T>(a, TG, cb, kx, step, active, col, L, t, tws, v);
+                target_sum<N, E, T, !CONS>(a, TG, cb, cur, cur, xv, false, parity, active, col, L, t, tws, v);
+                if (TG.noisy) stored_noise<N, E, T, !CONS>(a, TG, cb, kx, step, active, col, L, t, tws, v);
                 constraint_solve<E>(a, TG, cb, v);
                 if (edge_wg) hermitian_col<N, E, T>(v, L, t, edge);
                 if (active) {
@@ -572,7 +572,7 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
             }
         // ---- phi (setDynamic_k, src/field_kernels.cu:199-227)
         float2 acc[E];
-        target_sum<N, E, T>(a, a.tgt[0], cb, cur, cur, xv, pre, parity, active, col, L, t, tws, acc);
+        target_sum<N, E, T, !CONS>(a, a.tgt[0], cb, cur, cur, xv, pre, parity, active, col, L, t, tws, acc);
         float2 eta[E];
 #pragma unroll
         for (int e = 0; e < E; e++) eta[e] = make_float2(0.f, 0.f);
@@ -581,8 +581,8 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
             // the transforms of the terms
 #pragma unroll
             for (int e = 0; e < E; e++) eta[e] = active ? gload(a.eta, col + t + e * T) : make_float2(0.f, 0.f);
-            fft_line<N, E, -1, N>(eta, L, t, tws);
-            __syncthreads();
+            fft_line<N, E, -1, N, !CONS, (T <= 64)>(eta, L, t, tws);
+            line_sync<(T <= 64)>();
         } else if (a.noise_mode == NOISE_SPECTRAL) {
             // ky and ky + N/2 are elements e and e + E/2 of this lane
 #pragma unroll
@@ -622,7 +622,7 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
             const FusedTarget &TG = a.tgt[1 + ci];
             if (XDYN && TG.fresh) continue;
             float2 v[E];
-            target_sum<N, E, T>(a, TG, cb, cur, nw, xv, false, parity, active, col, L, t, tws, v);
+            target_sum<N, E, T, !CONS>(a, TG, cb, cur, nw, xv, false, parity, active, col, L, t, tws, v);
             if (XDYN && TG.dyn) {
                 const float2 *__restrict__ uin = a.cs[ci][parity];
 #pragma unroll
@@ -630,7 +630,7 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
                     const float2 u0 = active ? gload(uin, col + t + e * T) : make_float2(0.f, 0.f);
                     v[e] = make_float2(u0.x + a.u.dt * v[e].x, u0.y + a.u.dt * v[e].y);
                 }
-                if (TG.noisy) stored_noise<N, E, T>(a, TG, cb, kx, step, active, col, L, t, tws, v);
+                if (TG.noisy) stored_noise<N, E, T, !CONS>(a, TG, cb, kx, step, active, col, L, t, tws, v);
                 if (TG.nimp > 0) {
                     float rD[E];
                     pool_implicit_col<E>(a.pool, TG.imp0, TG.nimp, a.u.dt, cb, rD);
@@ -648,7 +648,7 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
                 for (int e = 0; e < E; e++) v[e] = active ? gload(kin, col + t + e * T) : make_float2(0.f, 0.f);
                 constraint_solve<E>(a, TG, cb, v);
             } else {
-                if (XDYN && TG.noisy) stored_noise<N, E, T>(a, TG, cb, kx, step, active, col, L, t, tws, v);
+                if (XDYN && TG.noisy) stored_noise<N, E, T, !CONS>(a, TG, cb, kx, step, active, col, L, t, tws, v);
                 constraint_solve<E>(a, TG, cb, v);
             }
             if (edge_wg) hermitian_col<N, E, T>(v, L, t, edge);
@@ -657,11 +657,11 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
 #pragma unroll
                 for (int e = 0; e < E; e++) gstore(cout, col + t + e * T, v[e]);
             }
-            if (TG.z >= 0) z_out<N, E, T>(a, TG.z, g, cb, v, false, blk.line, active, col, L, t, tws);
+            if (TG.z >= 0) z_out<N, E, T, !CONS>(a, TG.z, g, cb, v, false, blk.line, active, col, L, t, tws);
             if (XDYN && TG.dyn)
                 for (int f = 0; f < a.nreal; f++)
                     if (a.zsrc[f] < 0 && a.lbase[f] == ci)
-                        lz_out<N, E, T>(a, f, g, cb, v, false, blk.line, active, col, L, t, tws);
+                        lz_out<N, E, T, !CONS>(a, f, g, cb, v, false, blk.line, active, col, L, t, tws);
         }
     } else {
         // prologue: the real fields of the current state
@@ -674,17 +674,17 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
             float2 v[E];
 #pragma unroll
             for (int e = 0; e < E; e++) v[e] = active ? gload(a.cs[ci][0], col + t + e * T) : make_float2(0.f, 0.f);
-            if (z >= 0) z_out<N, E, T>(a, z, g, cb, v, a.zero[z] != 0, blk.line, active, col, L, t, tws);
+            if (z >= 0) z_out<N, E, T, !CONS>(a, z, g, cb, v, a.zero[z] != 0, blk.line, active, col, L, t, tws);
             if (dyn)
                 for (int f = 0; f < a.nreal; f++)
                     if (a.zsrc[f] < 0 && a.lbase[f] == ci)
-                        lz_out<N, E, T>(a, f, g, cb, v, a.zero[f] != 0, blk.line, active, col, L, t, tws);
+                        lz_out<N, E, T, !CONS>(a, f, g, cb, v, a.zero[f] != 0, blk.line, active, col, L, t, tws);
         }
     }
     // real fields that are phi or derived from it: L_f phi_{m+1}
     for (int f = 0; f < a.nreal; f++) {
         if (a.zsrc[f] >= 0 || a.lbase[f] >= 0) continue;
-        lz_out<N, E, T>(a, f, g, cb, nw, !upd && a.zero[f] != 0, blk.line, active, col, L, t, tws);
+        lz_out<N, E, T, !CONS>(a, f, g, cb, nw, !upd && a.zero[f] != 0, blk.line, active, col, L, t, tws);
     }
     if (slot) {
         __syncthreads();
@@ -939,10 +939,10 @@ __global__ __launch_bounds__((FRowCfg<NC, LPB, MAXR>::THREADS)) void k_f3rows(co
                 ya[e] = (active && NC - k <= km) ? gload(z, NC - k) : make_float2(0.f, 0.f);
             }
             c2r_prep<NC, E, T>(xa, ya, t, v, tws);
-            fft_line<NC, E, +1, 2 * NC>(v, L, t, tws);
+            fft_line<NC, E, +1, 2 * NC, false, (T <= 64)>(v, L, t, tws);
 #pragma unroll
             for (int e = 0; e < E; e++) r[f][e] = cscale(v[e], a.scale);
-            __syncthreads();
+            line_sync<(T <= 64)>();
         }
     }
     for (int gi = 0; gi < a.ngroups; gi++) {
@@ -1098,12 +1098,12 @@ __global__ __launch_bounds__((F3zCfg<N, TWD>::THREADS)) __attribute__((amdgpu_wa
                 for (int e = 0; e < E; e++)
                     xv[e] = active ? gload(a.X[gi], base + (long)(t + e * T) * plane) : make_float2(0.f, 0.f);
             }
-            fft_line<N, E, -1, N>(xv, L, t, tws);
+            fft_line<N, E, -1, N, true, (TWD * T <= 64)>(xv, L, t, tws);
             const TermRefDev &GM = a.gmul[gi];
             multiplier_line<E>(a.pool, GM.pres0, GM.npres, GM.mult_i, lb, m);
 #pragma unroll
             for (int e = 0; e < E; e++) acc[e] = cadd(acc[e], apply_mult(m[e], GM.mult_i, xv[e]));
-            __syncthreads();
+            line_sync<(TWD * T <= 64)>();
         }
         // single-field and constant terms (phi or a derived field of phi)
         for (int ti = 0; ti < TG.nterms; ti++) {
@@ -1187,13 +1187,13 @@ __global__ __launch_bounds__((F3zCfg<N, TWD>::THREADS)) __attribute__((amdgpu_wa
                 const float2 w = LM.npres > 0 ? apply_mult(lm[e], LM.mult_i, nw[e]) : nw[e];
                 v[e] = keep ? w : make_float2(0.f, 0.f);
             }
-            fft_line<N, E, +1, N>(v, L, t, tws);
+            fft_line<N, E, +1, N, true, (TWD * T <= 64)>(v, L, t, tws);
             if (active && kx <= a.kmax[f]) {
                 float2 *__restrict__ w = a.W[f];
 #pragma unroll
                 for (int e = 0; e < E; e++) gstore(w, base + (long)(t + e * T) * plane, v[e]);
             }
-            __syncthreads();
+            line_sync<(TWD * T <= 64)>();
         }
     }
     if (slot) {

@@ -136,7 +136,7 @@ __global__ __launch_bounds__(256) void k_rows_r2c(SrcArgs src, float2 *__restric
     float2 v[E];
 #pragma unroll
     for (int e = 0; e < E; e++) v[e] = active ? load_src<KIND>(src, row, t + e * T, NC) : make_float2(0.f, 0.f);
-    fft_line<NC, E, -1>(v, L, t, tw);
+    fft_line<NC, E, -1, TW_N, true, (T <= 64)>(v, L, t, tw);
     if (!active) return;
     float2 *o = out + row * (NC + 1);
 #pragma unroll
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(256) void k_rows_c2r(const float2 *__restrict__ in,
         const float2 xo = cmul(csub(X, Yc), tw_lookup<+1>(tw, k, 2 * NC));
         v[e] = cadd(xe, mul_i<+1>(xo));
     }
-    fft_line<NC, E, +1>(v, lds + l * C::LL, t, tw);
+    fft_line<NC, E, +1, TW_N, false, (T <= 64)>(v, lds + l * C::LL, t, tw);
     if (!active) return;
     float2 *o = reinterpret_cast<float2 *>(out + row * (2L * NC));
 #pragma unroll
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(512) void k_cols(const float2 *__restrict__ in, flo
             if (!keep) v[e] = make_float2(0.f, 0.f);
         }
     }
-    fft_line<N, E, DIR>(v, lds + c * C::LL, t, tw);
+    fft_line<N, E, DIR, TW_N, false, (TWD * T <= 64)>(v, lds + c * C::LL, t, tw);
     if (!active) return;
     float2 *dst = out + b * bstride + col;
 #pragma unroll
