@@ -121,7 +121,7 @@ def bench_ensemble(args, C):
         systems.append(s)
     npts = sum(n * n for n, _, _ in members)
     # plain path: a sample of 8 systems per size, each alone
-    plain_t, plain_pts = 0.0, 0
+    plain_t = 0.0
     k_plain = max(10, args.steps // 10)
     for size in (64, 128, 256, 512):
         idx = [i for i, m in enumerate(members) if m[0] == size][:8]
@@ -130,7 +130,6 @@ def bench_ensemble(args, C):
             systems[i].synchronize()
             ms = systems[i].time_steps(k_plain)
             plain_t += ms / 1e3 * (sum(1 for m in members if m[0] == size) / len(idx))
-            plain_pts += 0
     plain_value = npts * k_plain / plain_t
     batch = C.Batch(graph_steps=args.graph)
     for s in systems:
@@ -138,15 +137,50 @@ def bench_ensemble(args, C):
     batch.profile_kernel(True)
     print("bench: ensemble of %d instances, %.1fM points, noise %s, graph %d" %
           (len(members), npts / 1e6, args.noise, args.graph), file=sys.stderr, flush=True)
+    setup = _batch_setup(batch, args)
     batch.step(args.warmup)
     batch.synchronize()
     print(batch.schedule(), file=sys.stderr, end="", flush=True)
     ms = batch.time_steps(args.steps)
     value = npts * args.steps / (ms / 1e3)
     kernel = batch.kernel_time()
-    return value, ms, npts, {"plain_value": plain_value, "speedup_vs_plain": value / plain_value,
-                             "instances": len(members), "points": npts, "noise": args.noise,
-                             "graph_steps": args.graph}, kernel
+    extra = {"plain_value": plain_value, "speedup_vs_plain": value / plain_value,
+             "instances": len(members), "points": npts, "noise": args.noise,
+             "graph_steps": args.graph, "setup_steps": setup}
+    # the same ensemble with the reference's real-space noise draw (Philox
+    # per cell, then the forward transform), timed the same way
+    other = "real" if args.noise == "spectral" else "spectral"
+    for s in systems:
+        s.set_noise_mode(0 if other == "real" else 1)
+    _batch_setup(batch, args)
+    batch.step(args.warmup)
+    batch.synchronize()
+    ms2 = batch.time_steps(args.steps)
+    extra["value_%s_noise" % other] = npts * args.steps / (ms2 / 1e3)
+    extra["ms_per_step_%s_noise" % other] = ms2 / args.steps
+    batch.close()
+    for s in systems:
+        s.close()
+    return value, ms, npts, extra, kernel
+
+
+def _batch_setup(batch, args, min_s=0.3):
+    """Untimed set-up before the warm-up steps: the batch's first run builds
+    its tables, runs the autotuner, and (graph_steps >= 2) captures and
+    instantiates the K-step hipGraph, which only a run of at least K steps
+    does; then runs of K steps until min_s seconds have passed, so the timed
+    steps find the device at its sustained clocks rather than ramping up
+    from idle.  Returns the number of steps it advanced."""
+    k = args.graph if args.graph >= 2 else 1
+    batch.step(k)
+    batch.synchronize()
+    setup = k
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < min_s and setup < 4000:
+        batch.step(k)
+        batch.synchronize()
+        setup += k
+    return setup
 
 
 # HBM bytes per launch of the dominant kernel of each workload (read + written),
@@ -310,15 +344,19 @@ def cpu_baseline_ensemble_all_cores(members, budget_s=8.0):
     all W workers run, ensemble time = their sum over the instances / W.
     Runs before the GPU is touched, from spawned processes."""
     import multiprocessing as mp
-    workers = max(1, min(16, os.cpu_count() or 1))
+    # at most 16 worker processes: the host-CPU share of one GPU on the
+    # machines this runs on, whatever nproc reports there
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    workers = max(1, min(16, nproc))
     with mp.get_context("spawn").Pool(workers) as pool:
         samples = pool.map(_cpu_sample, [budget_s] * workers)
     per = {key: sum(p[key] for p in samples) / workers for key in samples[0]}
     total_t = sum(per[(n, lam != 0.0)] for n, _, lam in members) / workers
     npts = sum(n * n for n, _, _ in members)
     return {"value": npts / total_t, "unit": "grid-point-steps/s", "cores": workers, "kind": "port",
-            "sample": "the 1-core sample run by %d processes at once, instances dealt round-robin, %s" %
-                      (workers, _cpu_name())}
+            "nproc": nproc, "cpu_count": os.cpu_count(),
+            "sample": "the 1-core sample run by %d processes at once (nproc %d), instances dealt round-robin, %s" %
+                      (workers, nproc, _cpu_name())}
 
 
 def _cpu_name():
@@ -363,6 +401,8 @@ def main():
                     help="ensemble noise draw: real-space Philox + R2C, or spectral-space Philox")
     ap.add_argument("--graph", type=int, default=20, help="ensemble: steps per hipGraph (0 = direct launches)")
     ap.add_argument("--jobs", type=int, default=32, help="serve: number of jobs")
+    ap.add_argument("--serve-jobs", type=int, default=32,
+                    help="ensemble: also serve this many model-H jobs and report p50/p99 (0 = skip)")
     ap.add_argument("--load", type=float, default=0.8, help="serve: arrival rate / measured capacity")
     ap.add_argument("--budget-gib", type=float, default=4.0, help="serve: device-memory budget")
     ap.add_argument("--quantum", type=int, default=100, help="serve: steps per scheduling round")
@@ -414,6 +454,16 @@ def main():
                                                      "path": "batched fused step"}, **extra),
                "roofline": roofline(value, kernel, "8 B per grid point of the timed launch (the state update)",
                                     "ensemble")}
+        if args.serve_jobs > 0:
+            # the metric's latency half: a served model-H stream (Poisson
+            # arrivals at 80% of measured capacity) run after the ensemble
+            sargs = argparse.Namespace(**vars(args))
+            sargs.jobs = args.serve_jobs
+            svalue, sms, sextra, _ = bench_serve(sargs, C)
+            out["serve"] = {"value": svalue, "unit": "grid-point-steps/s", "workload": WORKLOADS["serve"]["name"],
+                            "p50_tts_s": sextra["p50_tts_s"], "p99_tts_s": sextra["p99_tts_s"],
+                            "jobs": sextra["jobs"], "load": sextra["load"], "capacity": sextra["capacity"],
+                            "wall_s": sextra["wall_s"]}
         if not args.no_cpu:
             out["cpu_baseline"] = cpu_baseline_ensemble(ensemble_members())
             out["cpu_baseline_all_cores"] = all_cores
