@@ -342,6 +342,10 @@ class Batch:
         check(self.L.cupss_batch_add(self.h, system.h))
         self.members.append(system)
 
+    def remove(self, system):
+        check(self.L.cupss_batch_remove(self.h, system.h))
+        self.members.remove(system)
+
     def step(self, n=1):
         check(self.L.cupss_batch_advance(self.h, n))
 

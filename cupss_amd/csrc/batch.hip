@@ -81,7 +81,13 @@ struct FusedBatch {
     FusedInst *d = nullptr;
     size_t dcap = 0;
     FusedInst *pinned = nullptr;  // staging of the descriptor upload
+    std::vector<char> dirty;      // descriptors that differ from the device's copy
     bool built = false;
+
+    // the last run's transpose back to the natural layout, deferred (see
+    // settle and flush): pending, and the parity of that run's length
+    bool settle_pending = false;
+    int settle_fin = 0;
 
     bool autotune = true;
     bool tuned = false;
@@ -392,34 +398,44 @@ static int describe(cupss_system s, FusedInst &a, bool natural = false)
 }
 
 // the per-run parts of a descriptor: buffers, step counter, step-0 flags
-static void run_view(cupss_system s, bool elementwise, FusedInst &a)
+// the per-run fields of an instance's descriptor; true when any changed
+// (step_base: the device copy advances with the steps, see run)
+static bool run_view(cupss_system s, bool elementwise, FusedInst &a)
 {
+    bool changed = false;
+    auto set = [&changed](auto &field, auto value) {
+        if (field != value) {
+            field = value;
+            changed = true;
+        }
+    };
     FieldRec &F = s->f[s->plan.phi];
     if (elementwise) {
-        a.spec[0] = F.spec[F.cur];
-        a.spec[1] = F.spec[F.cur ^ 1];
-        a.nat = nullptr;
+        set(a.spec[0], F.spec[F.cur]);
+        set(a.spec[1], F.spec[F.cur ^ 1]);
+        set(a.nat, (float2 *)nullptr);
     } else {
         const int b = s->resume_now ? s->cont_ft : 0;
-        a.spec[0] = s->ft[b];
-        a.spec[1] = s->ft[b ^ 1];
-        a.nat = F.spec[F.cur];
+        set(a.spec[0], s->ft[b]);
+        set(a.spec[1], s->ft[b ^ 1]);
+        set(a.nat, F.spec[F.cur]);
     }
-    a.resume = !elementwise && s->resume_now ? 1 : 0;
-    a.step_base = s->step;
-    a.cont = s->cont_now ? 1 : 0;
+    set(a.resume, !elementwise && s->resume_now ? 1 : 0);
+    set(a.step_base, (uint64_t)s->step);
+    set(a.cont, s->cont_now ? 1 : 0);
     const int b0 = s->cont_now ? s->cont_buf : 0;
     for (int c = 0; c < a.ncons; c++) {
         FieldRec &C = s->f[s->plan.cons[c]];
-        a.cnat[c] = C.spec[C.cur];
-        a.cs[c][0] = s->fc[c][b0];
-        a.cs[c][1] = s->fc[c][b0 ^ 1];
+        set(a.cnat[c], C.spec[C.cur]);
+        set(a.cs[c][0], s->fc[c][b0]);
+        set(a.cs[c][1], s->fc[c][b0 ^ 1]);
     }
-    for (int k = 0; k < a.nreal; k++) a.zero[k] = step0_zero(s, s->plan.real[k]) ? 1 : 0;
+    for (int k = 0; k < a.nreal; k++) set(a.zero[k], step0_zero(s, s->plan.real[k]) ? 1 : 0);
     for (int d = 0; d < a.nder; d++) {
         FieldRec &D = s->f[s->plan.derived[d]];
-        a.dout[d] = D.spec[D.cur];
+        set(a.dout[d], D.spec[D.cur]);
     }
+    return changed;
 }
 
 // table of (instance, first line) items in XCD-aware order
@@ -559,6 +575,7 @@ static int build(FusedBatch &B)
         B.gens.push_back(s->gen);
         B.vgens.push_back(s->vgen);
     }
+    B.dirty.assign(B.h.size(), 1);
     B.built = true;
     return CUPSS_OK;
 }
@@ -571,22 +588,40 @@ static bool stale_tables(const FusedBatch &B)
     return false;
 }
 
-// upload the per-run descriptors.  The device table is rewritten in stream
-// order, after the previous run's kernels; the staging buffer may still feed
-// the previous upload, so only that copy (which ran at the start of the
-// previous run) is waited for -- the host does not wait for the steps.
+// upload the per-run descriptors that changed (a run that follows a run of
+// an even number of steps usually changes none).  The device table is
+// rewritten in stream order, after the previous run's kernels; the staging
+// buffer may still feed the previous upload, so only that copy is waited
+// for -- the host does not wait for the steps.
 static int refresh(FusedBatch &B)
 {
-    if (B.staged) HIP_TRY(hipEventSynchronize(B.staged));
-    for (size_t i = 0; i < B.sys.size(); i++) {
+    const size_t n = B.h.size();
+    if (B.dirty.size() != n) B.dirty.assign(n, 1);
+    bool any = false;
+    for (size_t i = 0; i < n; i++) {
         if (B.vgens[i] != B.sys[i]->vgen) {  // new prefactor values, same structure: new pool
             if (int r = describe(B.sys[i], B.h[i])) return r;
             B.vgens[i] = B.sys[i]->vgen;
+            B.dirty[i] = 1;
         }
-        run_view(B.sys[i], B.is_elem[i], B.h[i]);
+        if (run_view(B.sys[i], B.is_elem[i], B.h[i])) B.dirty[i] = 1;
+        any = any || B.dirty[i];
     }
-    std::memcpy(B.pinned, B.h.data(), sizeof(FusedInst) * B.h.size());
-    HIP_TRY(hipMemcpyAsync(B.d, B.pinned, sizeof(FusedInst) * B.h.size(), hipMemcpyHostToDevice, B.stream));
+    if (!any) return CUPSS_OK;
+    if (B.staged) HIP_TRY(hipEventSynchronize(B.staged));
+    for (size_t i = 0; i < n;) {
+        if (!B.dirty[i]) {
+            i++;
+            continue;
+        }
+        size_t j = i;
+        for (; j < n && B.dirty[j]; j++) {
+            std::memcpy(B.pinned + j, &B.h[j], sizeof(FusedInst));
+            B.dirty[j] = 0;
+        }
+        HIP_TRY(hipMemcpyAsync(B.d + i, B.pinned + i, sizeof(FusedInst) * (j - i), hipMemcpyHostToDevice, B.stream));
+        i = j;
+    }
     if (!B.staged) HIP_TRY(hipEventCreateWithFlags(&B.staged, hipEventDisableTiming));
     HIP_TRY(hipEventRecord(B.staged, B.stream));
     return CUPSS_OK;
@@ -848,11 +883,19 @@ static int retable(FusedBatch &B)
 // wrote both in place.  Member streams then wait for the batch stream.
 static int settle(FusedBatch &B, int nsteps)
 {
-    if (B.ntb > 0) HIP_TRY(fused_settle_launch(B.d, B.tb, B.ntb, nsteps & 1, B.stream));
-    if (!B.done) HIP_TRY(hipEventCreateWithFlags(&B.done, hipEventDisableTiming));
-    HIP_TRY(hipEventRecord(B.done, B.stream));
+    // the transpose back is deferred: a following run resumes from the
+    // column-major state; anything else that touches a member flushes it
+    B.settle_pending = true;
+    B.settle_fin = nsteps & 1;
+    bool waits = false;
+    for (cupss_system s : B.sys) waits = waits || s->stream != B.stream;
+    if (waits) {
+        if (!B.done) HIP_TRY(hipEventCreateWithFlags(&B.done, hipEventDisableTiming));
+        HIP_TRY(hipEventRecord(B.done, B.stream));
+    }
     for (size_t i = 0; i < B.sys.size(); i++) {
         cupss_system s = B.sys[i];
+        s->pending = &B;
         const FusedPlan &P = s->plan;
         FieldRec &F = s->f[P.phi];
         if (B.is_elem[i]) F.cur ^= (nsteps & 1);
@@ -878,11 +921,54 @@ static int settle(FusedBatch &B, int nsteps)
     return CUPSS_OK;
 }
 
+// the deferred transpose back of the last run: every transform instance's
+// spectrum, stored fields and derived fields in natural layout again
+static int flush(FusedBatch &B)
+{
+    if (!B.settle_pending) return CUPSS_OK;
+    B.settle_pending = false;
+    if (B.ntb > 0) HIP_TRY(fused_settle_launch(B.d, B.tb, B.ntb, B.settle_fin, B.stream));
+    bool waits = false;
+    for (cupss_system s : B.sys) {
+        if (s->pending == &B) s->pending = nullptr;
+        waits = waits || s->stream != B.stream;
+    }
+    if (waits) {
+        if (!B.done) HIP_TRY(hipEventCreateWithFlags(&B.done, hipEventDisableTiming));
+        HIP_TRY(hipEventRecord(B.done, B.stream));
+        for (cupss_system s : B.sys)
+            if (s->stream != B.stream) HIP_TRY(hipStreamWaitEvent(s->stream, B.done, 0));
+    }
+    return CUPSS_OK;
+}
+
+int fused_flush(cupss_system s)
+{
+    if (!s || !s->pending) return CUPSS_OK;
+    return flush(*static_cast<FusedBatch *>(s->pending));
+}
+
 static int run(FusedBatch &B, int nsteps)
 {
     if (nsteps <= 0) return CUPSS_OK;
+    // a member whose state another batch's run left column-major: written
+    // back first
+    for (cupss_system s : B.sys)
+        if (s->pending && s->pending != &B)
+            if (int r = fused_flush(s)) return r;
     for (cupss_system s : B.sys)
         if (s->stream != B.stream) HIP_TRY(hipStreamSynchronize(s->stream));
+    // this batch's own deferred transpose: needed only where a transform
+    // instance cannot resume from the column-major state (tables rebuilt,
+    // autotuning, a state changed since), and before the descriptors and
+    // tables it reads change
+    if (B.settle_pending) {
+        bool need = stale_tables(B) || (B.autotune && !B.tuned);
+        for (size_t i = 0; i < B.sys.size() && !need; i++)
+            need = !B.is_elem[i] && (B.sys[i]->cont_step != B.sys[i]->step || B.sys[i]->cont_gen != B.sys[i]->gen);
+        if (need)
+            if (int r = flush(B)) return r;
+    }
     if (stale_tables(B))
         if (int r = build(B)) return r;
     // Systems with stored constraint fields: phase A/B of the run's first
@@ -941,6 +1027,13 @@ static int run(FusedBatch &B, int nsteps)
     }
     if (nsteps > done) {
         if (int r = launch_all(B, done, nsteps - done, done)) return r;
+        // the device's step counters follow the steps, so that the next
+        // run's descriptors need no upload for them
+        const int ni = (int)B.h.size();
+        hipLaunchKernelGGL(k_bump_steps, dim3((ni + 255) / 256), dim3(256), 0, B.stream, B.d, ni,
+                           (uint32_t)(nsteps - done));
+        HIP_TRY(hipGetLastError());
+        for (size_t i = 0; i < B.h.size(); i++) B.h[i].step_base += (uint64_t)(nsteps - done);
     }
     B.steps_run += nsteps;
     return settle(B, nsteps);
@@ -1158,10 +1251,29 @@ extern "C" int cupss_batch_create(void *hip_stream, cupss_batch *out)
     return CUPSS_OK;
 }
 
+// a member leaves the batch: its state written back, the batch's work on it
+// done, its own stream restored
+static int leave(cupss_batch b, cupss_system s)
+{
+    if (int r = flush(b->b)) return r;
+    HIP_TRY(hipStreamSynchronize(b->b.stream));
+    if (s->member_of == b) {
+        s->stream = s->home_stream;
+        s->home_stream = nullptr;
+        s->member_of = nullptr;
+    }
+    return CUPSS_OK;
+}
+
 extern "C" int cupss_batch_destroy(cupss_batch b)
 {
+    if (!b) return CUPSS_OK;
+    int rc = CUPSS_OK;
+    for (cupss_system s : b->b.sys)
+        if (int r = leave(b, s)) rc = r;
+    b->b.sys.clear();
     delete b;
-    return CUPSS_OK;
+    return rc;
 }
 
 extern "C" int cupss_batch_add(cupss_batch b, cupss_system s)
@@ -1172,11 +1284,17 @@ extern "C" int cupss_batch_add(cupss_batch b, cupss_system s)
     for (auto &F : s->f)
         if (F.real_hook || F.spec_hook || F.real_dev || F.spec_dev)
             return fail(CUPSS_ERR_UNSUPPORTED, "systems with callbacks cannot be batched");
+    if (s->member_of) return fail(CUPSS_ERR_STATE, "the system is already in a batch");
     if (fused_stale(s)) {
         // consume a stale dealiased copy with one general step, as a lone system would
         if (int r = general_steps(s, 1)) return r;
-        HIP_TRY(hipStreamSynchronize(s->stream));
     }
+    // its own queued work done, the member's work then goes to the batch stream
+    if (int r = fused_flush(s)) return r;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->home_stream = s->stream;
+    s->stream = b->b.stream;
+    s->member_of = b;
     b->b.sys.push_back(s);
     b->b.built = false;
     return CUPSS_OK;
@@ -1188,7 +1306,7 @@ extern "C" int cupss_batch_remove(cupss_batch b, cupss_system s)
     auto &v = b->b.sys;
     auto it = std::find(v.begin(), v.end(), s);
     if (it == v.end()) return fail(CUPSS_ERR_INVALID, "system is not in the batch");
-    HIP_TRY(hipStreamSynchronize(b->b.stream));
+    if (int r = leave(b, s)) return r;
     v.erase(it);
     b->b.built = false;
     return CUPSS_OK;
@@ -1315,8 +1433,10 @@ extern "C" int cupss_batch_advance(cupss_batch b, int nsteps)
 extern "C" int cupss_batch_time_advance(cupss_batch b, int nsteps, float *ms)
 {
     if (!b) return fail(CUPSS_ERR_INVALID, "null batch");
-    if (stale_tables(b->b))
+    if (stale_tables(b->b)) {  // host set-up outside the timed region
+        if (int r = flush(b->b)) return r;
         if (int r = build(b->b)) return r;
+    }
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));

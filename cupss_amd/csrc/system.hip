@@ -105,7 +105,9 @@ extern "C" int cupss_system_create(int sx, int sy, int sz, float dx, float dy, f
 extern "C" int cupss_system_destroy(cupss_system s)
 {
     if (!s) return CUPSS_OK;
+    if (s->member_of) cupss_batch_remove((cupss_batch)s->member_of, s);
     if (s->stream) hipStreamSynchronize(s->stream);
+    s->pending = nullptr;  // only the system's own batch can still hold it
     fused_release(s);
     if (s->gexec) hipGraphExecDestroy(s->gexec);
     if (s->graph) hipGraphDestroy(s->graph);
@@ -269,6 +271,7 @@ extern "C" int cupss_field_set_device_hooks(cupss_system s, int field, cupss_rea
 extern "C" int cupss_field_set_real(cupss_system s, int field, const float *host)
 {
     if (int r = check_field(s, field)) return r;
+    if (int r = fused_flush(s)) return r;
     s->cont_step = ~0ull;
     if (!host) return fail(CUPSS_ERR_INVALID, "null host pointer");
     // the dealiased copy keeps its old values (prepareProblem re-runs only
@@ -291,6 +294,7 @@ extern "C" int cupss_field_get_real(cupss_system s, int field, float *host)
 {
     if (int r = check_field(s, field)) return r;
     if (!host) return fail(CUPSS_ERR_INVALID, "null host pointer");
+    if (int r = fused_flush(s)) return r;
     FieldRec &F = s->f[field];
     HIP_TRY(c2r(s->g, F.spec[F.cur], 0, 1.0f / (float)s->g.n, s->rtmp, s->scratch, s->stream));
     HIP_TRY(hipMemcpyAsync(host, s->rtmp, sizeof(float) * s->g.n, hipMemcpyDeviceToHost, s->stream));
@@ -302,6 +306,7 @@ extern "C" int cupss_field_get_dealiased(cupss_system s, int field, float *host)
 {
     if (int r = check_field(s, field)) return r;
     if (!host) return fail(CUPSS_ERR_INVALID, "null host pointer");
+    if (int r = fused_flush(s)) return r;
     FieldRec &F = s->f[field];
     if (!F.dreal) {
         std::memset(host, 0, sizeof(float) * s->g.n);
@@ -316,6 +321,7 @@ extern "C" int cupss_field_get_dealiased(cupss_system s, int field, float *host)
 extern "C" int cupss_field_get_spectrum(cupss_system s, int field, float *host)
 {
     if (int r = check_field(s, field)) return r;
+    if (int r = fused_flush(s)) return r;
     FieldRec &F = s->f[field];
     HIP_TRY(hipMemcpyAsync(host, F.spec[F.cur], sizeof(float2) * s->g.nh, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -325,6 +331,7 @@ extern "C" int cupss_field_get_spectrum(cupss_system s, int field, float *host)
 extern "C" int cupss_field_set_spectrum(cupss_system s, int field, const float *host)
 {
     if (int r = check_field(s, field)) return r;
+    if (int r = fused_flush(s)) return r;
     s->cont_step = ~0ull;
     FieldRec &F = s->f[field];
     HIP_TRY(hipMemcpyAsync(F.spec[F.cur], host, sizeof(float2) * s->g.nh, hipMemcpyHostToDevice, s->stream));
@@ -337,6 +344,7 @@ extern "C" long cupss_system_half_bins(cupss_system s) { return s ? s->g.nh : 0;
 extern "C" int cupss_field_spectrum_device(cupss_system s, int field, void **dev_ptr)
 {
     if (int r = check_field(s, field)) return r;
+    if (int r = fused_flush(s)) return r;
     *dev_ptr = s->f[field].spec[s->f[field].cur];
     s->cont_step = ~0ull;  // the caller may write the spectrum
     return CUPSS_OK;
@@ -576,6 +584,7 @@ static int post_update(cupss_system s, int fi)
 // write the dealiased copy of a field whose copy is current but not stored
 int cupss::materialize(cupss_system s, int fi)
 {
+    if (int r = fused_flush(s)) return r;
     FieldRec &F = s->f[fi];
     if (F.dstate != 2 || !F.dreal || F.order <= 0) return CUPSS_OK;
     HIP_TRY(c2r(s->g, F.spec[F.cur], F.order, 1.0f / (float)s->g.n, F.dreal, s->scratch, s->stream));
@@ -635,6 +644,7 @@ static int run_phase(cupss_system s, int dynamic)
 
 int cupss::general_constraint_phase(cupss_system s)
 {
+    if (int r = fused_flush(s)) return r;
     for (int i = 0; i < (int)s->f.size(); i++)
         if (int r = materialize(s, i)) return r;
     return run_phase(s, 0);
@@ -642,6 +652,7 @@ int cupss::general_constraint_phase(cupss_system s)
 
 int cupss::general_steps(cupss_system s, int nsteps)
 {
+    if (int r = fused_flush(s)) return r;
     if (nsteps > 0) s->cont_step = ~0ull;
     for (int i = 0; i < (int)s->f.size(); i++)
         if (int r = materialize(s, i)) return r;
@@ -687,6 +698,7 @@ static bool graph_parity(cupss_system s)
 
 static int general_advance(cupss_system s, int nsteps)
 {
+    if (int r = fused_flush(s)) return r;
     const int K = s->graph_steps;
     bool noisy = false;
     for (auto &F : s->f) noisy = noisy || F.noisy;
@@ -1095,6 +1107,8 @@ extern "C" int cupss_system_set_step(cupss_system s, uint64_t step)
 extern "C" int cupss_system_set_stream(cupss_system s, void *hip_stream)
 {
     if (int r = check_sys(s)) return r;
+    if (s->member_of) return fail(CUPSS_ERR_STATE, "a batch member runs on the batch's stream: remove it first");
+    if (int r = fused_flush(s)) return r;
     HIP_TRY(hipStreamSynchronize(s->stream));
     if (s->own_stream) hipStreamDestroy(s->stream);
     s->own_stream = false;

@@ -113,6 +113,16 @@ struct cupss_system_t {
     uint64_t step = 0;
     int device = 0;
     void *batch = nullptr;    // internal one-instance fused batch
+    // The fused batch (internal or a cupss_batch's) whose last run left this
+    // system's state column-major only: its transpose back to the natural
+    // layout is deferred until something reads or writes the state outside
+    // such a run (fused_flush), so runs that follow runs pay none.
+    void *pending = nullptr;
+    // Member of a cupss_batch: the system's work goes to the batch's stream
+    // while it is a member (ordered with the batch's runs without per-run
+    // synchronisation); its own stream is kept here and restored on removal.
+    void *member_of = nullptr;
+    hipStream_t home_stream = nullptr;
     uint64_t gen = 1;         // bumped by every change that alters the step program
     uint64_t vgen = 1;        // bumped by prefactor value changes that keep the program's structure
     // general path: K steps captured once in a hipGraph and replayed
@@ -140,6 +150,9 @@ bool fused_stale(cupss_system s);
 // fused path (batch.hip)
 int fused_steps(cupss_system s, int nsteps);
 void fused_release(cupss_system s);
+// write back the natural-layout state that a fused run deferred (see
+// cupss_system_t::pending); no-op when nothing is pending
+int fused_flush(cupss_system s);
 int fused_profile(cupss_system s, int on);
 int fused_kernel_time(cupss_system s, float *avg_ms, double *points, char *name, int len);
 }  // namespace cupss

@@ -102,6 +102,44 @@ def test_split_runs_equal_one_run():
         assert np.array_equal(ref, d.get_real(idd[key]))
 
 
+def test_members_stepped_alone_and_removed():
+    """A member advanced on its own between batch runs, a member read and
+    one removed in the middle: every state ends bit-identical to the same
+    system advanced alone.  (Batch runs defer the transpose of the state
+    back to the natural layout until something reads or advances a member
+    elsewhere; these are the paths that must write it back first.)"""
+    specs = [spec_kpz(64, nu=1.2, lam=0.9, D=0.01, dt=0.005, seed=61),
+             spec_cahn_hilliard((128, 128, 1), seed=62), spec_model_h(64, seed=63)]
+    keys = ["h", "phi", "phi"]
+    for graph in (0, 4):
+        batch = C.Batch(graph_steps=graph)
+        mem = []
+        for sp in specs:
+            g, ids = make(sp)
+            batch.add(g)
+            mem.append((g, ids))
+        batch.step(6)
+        mem[0][0].step(3)                      # alone, while still a member
+        batch.step(4)
+        mid = mem[1][0].get_real(mem[1][1]["phi"])
+        batch.step(2)
+        batch.remove(mem[2][0])
+        mem[2][0].step(5)                      # alone after leaving
+        batch.step(4)
+        counts = [6 + 3 + 4 + 2 + 4, 6 + 4 + 2 + 4, 6 + 4 + 2 + 5]
+        for (g, ids), sp, key, n in zip(mem, specs, keys, counts):
+            ref, rids = make(sp)
+            ref.step(n)
+            got = g.get_real(ids[key])
+            print("  graph %d %s: %d steps, max|diff| %.3e" % (graph, g.dims, n, float(np.max(np.abs(got - ref.get_real(rids[key]))))))
+            assert g.step_count == n
+            assert np.array_equal(got, ref.get_real(rids[key]))
+        ref, rids = make(specs[1])
+        ref.step(10)
+        assert np.array_equal(mid, ref.get_real(rids["phi"]))
+        batch.close()
+
+
 def ew_structure_factor(n, nu, D, dt, noise_mode, steps, burn, every=1):
     sp = spec_kpz(n, nu=nu, lam=0.0, D=D, dt=dt, seed=11)
     g, ids = make(sp, noise_mode)
