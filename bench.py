@@ -140,10 +140,10 @@ def bench_ensemble(args, C):
     setup = _batch_setup(batch, args)
     batch.step(args.warmup)
     batch.synchronize()
-    print(batch.schedule(), file=sys.stderr, end="", flush=True)
     ms = batch.time_steps(args.steps)
     value = npts * args.steps / (ms / 1e3)
     kernel = batch.kernel_time()
+    print(batch.schedule(), file=sys.stderr, end="", flush=True)
     extra = {"plain_value": plain_value, "speedup_vs_plain": value / plain_value,
              "instances": len(members), "points": npts, "noise": args.noise,
              "graph_steps": args.graph, "setup_steps": setup}

@@ -66,6 +66,10 @@ struct FusedBatch {
         std::vector<int> insts;
         int lpb = 0, twd = 0;   // tile shapes
         std::vector<Tables> chunks;
+        // whole steps of an instance in one workgroup (fused_small_class):
+        // one launch per run of k steps over these instance ids
+        bool small = false;
+        int *ids = nullptr;
     };
     std::vector<Class> classes;
     std::vector<int> elem;      // elementwise instances
@@ -105,6 +109,7 @@ struct FusedBatch {
     int pclass = -1;
     unsigned long long *pts = nullptr;  // timestamp slots of up to PROF_SLOTS timed launches
     int pcount = 0;
+    int psteps = 1;  // steps one timed launch takes (a small-instance launch takes a run's k)
     static constexpr int PROF_SLOTS = 64;
 
     // independent work (size classes, the elementwise set) runs on up to
@@ -146,7 +151,10 @@ struct FusedBatch {
     void release_tables()
     {
         if (stream) hipStreamSynchronize(stream);  // launches in flight may still read the tables
-        for (auto &c : classes) free_tables(c.chunks);
+        for (auto &c : classes) {
+            free_tables(c.chunks);
+            hipFree(c.ids);
+        }
         classes.clear();
         hipFree(eb);
         eb = nullptr;
@@ -564,6 +572,12 @@ static int build(FusedBatch &B)
         default_shapes(c);
         if (c.lpb <= 0 || c.twd <= 0) return fail(CUPSS_ERR_UNSUPPORTED, "no tile shape for this size class");
         if (int r = class_tables(B, c, c.lpb, c.twd, B.chunk_bytes, c.chunks)) return r;
+        c.small = fused_small_class(c.nc, c.ny, c.wide);
+        if (c.small) {
+            if (hipMalloc(&c.ids, sizeof(int) * c.insts.size()) != hipSuccess)
+                return fail(CUPSS_ERR_NOMEM, "device allocation failed");
+            HIP_TRY(hipMemcpy(c.ids, c.insts.data(), sizeof(int) * c.insts.size(), hipMemcpyHostToDevice));
+        }
     }
     if (int r = upload_blocks(er, &B.eb, &B.neb)) return r;
     if (int r = upload_blocks(tiles, &B.tb, &B.ntb)) return r;
@@ -653,6 +667,15 @@ static int launch_class(FusedBatch &B, size_t ci, int n0, int k, int rel0, hipSt
 {
     auto &c = B.classes[ci];
     const bool timed = B.profile && (int)ci == B.pclass && c.chunks.size() == 1;
+    if (c.small) {
+        unsigned long long *ts = nullptr;
+        if (timed && B.pcount < FusedBatch::PROF_SLOTS) {
+            ts = B.pts + (size_t)(B.pcount++) * FUSED_TS_SLOTS * FUSED_TS_STRIDE;
+            B.psteps = k;
+        }
+        HIP_TRY(fused_small_launch(c.nc, c.ny, c.cons, B.d, c.ids, (int)c.insts.size(), n0, k, rel0, st, ts));
+        return CUPSS_OK;
+    }
     for (auto &t : c.chunks)
         for (int n = n0; n < n0 + k; n++) {
             HIP_TRY(fused_rows_launch(c.nc, c.lpb, c.wide, B.d, t.rb, t.nrb, (uint32_t)(n - rel0), st));
@@ -697,7 +720,14 @@ static int launch_all(FusedBatch &B, int n0, int k, int rel0)
         items.push_back({cost, -1});
     }
     const int L = std::min<int>(FusedBatch::MAX_LANES, (int)items.size());
-    std::sort(items.begin(), items.end(), [](const std::pair<double, int> &a, const std::pair<double, int> &b) {
+    // Work that takes all k steps in one launch on few workgroups (the
+    // small-instance classes, the elementwise set) is enqueued first, so it
+    // starts beside the per-step launches of the large classes instead of
+    // running alone after them; then the rest, longest first.
+    auto one_launch = [&B](int ci) { return ci < 0 || B.classes[ci].small; };
+    std::sort(items.begin(), items.end(), [&](const std::pair<double, int> &a, const std::pair<double, int> &b) {
+        const bool oa = one_launch(a.second), ob = one_launch(b.second);
+        if (oa != ob) return oa;
         return a.first > b.first;
     });
     std::vector<double> load(std::max(L, 1), 0.0);
@@ -735,12 +765,17 @@ static int launch_all(FusedBatch &B, int n0, int k, int rel0)
     return CUPSS_OK;
 }
 
-// the class the timing events follow: the most real points
+// the class the timing events follow: the most real points, of the classes
+// that launch a column pass per step if there are any
 static void pick_profile_class(FusedBatch &B)
 {
     B.pclass = -1;
+    B.psteps = 1;
     double best = -1;
+    bool per_step = false;
+    for (const auto &c : B.classes) per_step = per_step || !c.small;
     for (size_t ci = 0; ci < B.classes.size(); ci++) {
+        if (per_step && B.classes[ci].small) continue;
         double pts = 0;
         for (int i : B.classes[ci].insts) pts += (double)B.h[i].g.n;
         if (pts > best) {
@@ -787,6 +822,7 @@ static int tune(FusedBatch &B)
     Timer T;
     const int reps = 3;
     for (auto &c : B.classes) {
+        if (c.small) continue;  // fixed tiles, one launch per run
         // rows
         float best = 1e30f;
         int pick = c.lpb;
@@ -1379,7 +1415,8 @@ static int kernel_time(FusedBatch &B, float *avg_ms, double *points, char *name,
         if (h[2 * j + 1] < h[2 * j]) return fail(CUPSS_ERR_STATE, "timed launch did not record");
         sum += (double)(h[2 * j + 1] - h[2 * j]) / (double)khz;
     }
-    if (avg_ms) *avg_ms = (float)(sum / B.pcount);
+    // per step: a small-instance launch takes psteps steps
+    if (avg_ms) *avg_ms = (float)(sum / B.pcount / B.psteps);
     if (B.pclass == -2) {  // 3D: the z pass of the one system
         if (points) *points = (double)B.h[0].g.n;
         if (name && len > 0) std::snprintf(name, (size_t)len, "k_f3z<%d, %d>", B.h[0].g.sz, B.twd3);
@@ -1389,9 +1426,14 @@ static int kernel_time(FusedBatch &B, float *avg_ms, double *points, char *name,
     double pts = 0;
     for (int i : c.insts) pts += (double)B.h[i].g.n;
     if (points) *points = pts;
-    if (name && len > 0)
-        std::snprintf(name, (size_t)len, "k_fcols<%d, %d, %s, %s>", c.ny, c.twd, c.cons ? "true" : "false",
-                      c.cons == 2 ? "true" : "false");
+    if (name && len > 0) {
+        if (c.small)
+            std::snprintf(name, (size_t)len, "k_fsmall<%d, %d, %s, %s>", c.nc, c.ny, c.cons ? "true" : "false",
+                          c.cons == 2 ? "true" : "false");
+        else
+            std::snprintf(name, (size_t)len, "k_fcols<%d, %d, %s, %s>", c.ny, c.twd, c.cons ? "true" : "false",
+                          c.cons == 2 ? "true" : "false");
+    }
     return CUPSS_OK;
 }
 

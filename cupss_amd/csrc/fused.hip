@@ -148,25 +148,21 @@ __device__ __forceinline__ void r2c_tile(float2 (&v)[E], float2 *L, int t, const
     __syncthreads();
 }
 
+// Rows y0 .. y0+LPB-1 of one instance (the whole workgroup).  load_tw:
+// copy the twiddle table into tws first (otherwise it is there already).
+// Ends with a workgroup barrier: the tile and the rows written are free.
 template <int NC, int LPB, int MAXR>
-__global__ __launch_bounds__((FRowCfg<NC, LPB, MAXR>::THREADS)) void k_frows(const FusedInst *__restrict__ insts,
-                                                                          const FusedBlock *__restrict__ blocks,
-                                                                          uint32_t step_rel,
-                                                                          const float2 *__restrict__ sets)
+__device__ __forceinline__ void frows_tile(const FusedInst &a, int y0, uint32_t step_rel,
+                                           const float2 *__restrict__ sets, bool load_tw, float2 *tws, float2 *tile)
 {
     using C = FRowCfg<NC, LPB, MAXR>;
     constexpr int E = C::E, T = C::T, RS = C::RS, TH = C::THREADS;
     static_assert(T % 2 == 0 && E % 2 == 0, "noise lanes are paired");
-    __shared__ float2 tws[2 * NC];
-    __shared__ float2 tile[LPB * RS];
-    const FusedBlock blk = blocks[blockIdx.x];
-    if (blk.inst < 0) return;
-    const FusedInst &a = insts[blk.inst];
-    const int sy = a.g.sy, y0 = blk.line;
+    const int sy = a.g.sy;
     const int l = threadIdx.x / T, t = threadIdx.x % T;
     float2 *L = tile + l * RS;
     const int nreal = a.nreal;
-    load_twiddles<2 * NC>(tws, sets);
+    if (load_tw) load_twiddles<2 * NC>(tws, sets);
 
     float2 r[MAXR][E];
 #pragma unroll
@@ -260,6 +256,20 @@ __global__ __launch_bounds__((FRowCfg<NC, LPB, MAXR>::THREADS)) void k_frows(con
         tile_store<NC, LPB, RS, TH>(eta, tile, sy, y0);
         __syncthreads();  // the tile is the next source's transform LDS
     }
+}
+
+template <int NC, int LPB, int MAXR>
+__global__ __launch_bounds__((FRowCfg<NC, LPB, MAXR>::THREADS)) void k_frows(const FusedInst *__restrict__ insts,
+                                                                          const FusedBlock *__restrict__ blocks,
+                                                                          uint32_t step_rel,
+                                                                          const float2 *__restrict__ sets)
+{
+    using C = FRowCfg<NC, LPB, MAXR>;
+    __shared__ float2 tws[2 * NC];
+    __shared__ float2 tile[LPB * C::RS];
+    const FusedBlock blk = blocks[blockIdx.x];
+    if (blk.inst < 0) return;
+    frows_tile<NC, LPB, MAXR>(insts[blk.inst], blk.line, step_rel, sets, true, tws, tile);
 }
 
 // ------------------------------------------------------------ column kernel
@@ -502,26 +512,17 @@ struct FColCfg {
 
 // CONS: the instances may have stored constraint fields (compiled out
 // otherwise, which keeps the kernel within 128 VGPRs: 4 waves per SIMD)
+// Columns line .. line+TWD-1 of one instance (the whole workgroup); mode
+// 0: prologue (the real fields of the current state), 1: step.  load_tw as
+// for frows_tile.  Each column's LDS region is used by its own lanes only.
 template <int N, int TWD, bool CONS, bool XDYN>
-__global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const FusedInst *__restrict__ insts,
-                                                                   const FusedBlock *__restrict__ blocks,
-                                                                   int parity, uint32_t step_rel, int mode,
-                                                                   const float2 *__restrict__ sets,
-                                                                   unsigned long long *__restrict__ ts)
+__device__ __forceinline__ void fcols_tile(const FusedInst &a, int line, int parity, uint32_t step_rel, int mode,
+                                           const float2 *__restrict__ sets, bool load_tw, float2 *tws,
+                                           float2 *lds)
 {
     using C = FColCfg<N, TWD>;
     constexpr int E = C::E, T = C::T;
-    __shared__ float2 tws[N];
-    __shared__ float2 lds[TWD * C::LL];
-    const FusedBlock blk = blocks[blockIdx.x];
-    if (blk.inst < 0) return;
-    if (mode == 0 && insts[blk.inst].resume) return;  // Z_f of this state left by the previous run
-    // launch timing (ts != null): earliest workgroup start and latest end in
-    // ticks of the constant-rate wall clock, spread over FUSED_TS_SLOTS
-    // cache lines so that the workgroups' atomics do not contend
-    unsigned long long *slot = ts ? ts + (size_t)(blockIdx.x % FUSED_TS_SLOTS) * FUSED_TS_STRIDE : nullptr;
-    if (slot && threadIdx.x == 0) atomicMin(&slot[0], (unsigned long long)wall_clock64());
-    const FusedInst &a = insts[blk.inst];
+    const FusedBlock blk{0, line};
     const Geom &g = a.g;
     const int c = threadIdx.x / T, t = threadIdx.x % T;
     const int kx = blk.line + c;
@@ -541,8 +542,10 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
         cur[e] = active ? gload(sin, i) : make_float2(0.f, 0.f);
         xv[e] = (active && pre) ? gload(a.X[a.tgt[0].g0], i) : make_float2(0.f, 0.f);
     }
-    load_twiddles<N>(tws, sets);
-    __syncthreads();
+    if (load_tw) {
+        load_twiddles<N>(tws, sets);
+        __syncthreads();
+    }
 
     ColBins<E> cb;
     col_bins<E, T>(g, kx, t, cb);
@@ -686,10 +689,78 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
         if (a.zsrc[f] >= 0 || a.lbase[f] >= 0) continue;
         lz_out<N, E, T, !CONS>(a, f, g, cb, nw, !upd && a.zero[f] != 0, blk.line, active, col, L, t, tws);
     }
+}
+
+template <int N, int TWD, bool CONS, bool XDYN>
+__global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const FusedInst *__restrict__ insts,
+                                                                   const FusedBlock *__restrict__ blocks,
+                                                                   int parity, uint32_t step_rel, int mode,
+                                                                   const float2 *__restrict__ sets,
+                                                                   unsigned long long *__restrict__ ts)
+{
+    using C = FColCfg<N, TWD>;
+    __shared__ float2 tws[N];
+    __shared__ float2 lds[TWD * C::LL];
+    const FusedBlock blk = blocks[blockIdx.x];
+    if (blk.inst < 0) return;
+    if (mode == 0 && insts[blk.inst].resume) return;  // Z_f of this state left by the previous run
+    // launch timing (ts != null): earliest workgroup start and latest end in
+    // ticks of the constant-rate wall clock, spread over FUSED_TS_SLOTS
+    // cache lines so that the workgroups' atomics do not contend
+    unsigned long long *slot = ts ? ts + (size_t)(blockIdx.x % FUSED_TS_SLOTS) * FUSED_TS_STRIDE : nullptr;
+    if (slot && threadIdx.x == 0) atomicMin(&slot[0], (unsigned long long)wall_clock64());
+    fcols_tile<N, TWD, CONS, XDYN>(insts[blk.inst], blk.line, parity, step_rel, mode, sets, true, tws, lds);
     if (slot) {
         __syncthreads();
         if (threadIdx.x == 0) atomicMax(&slot[1], (unsigned long long)wall_clock64());
     }
+}
+
+// ------------------------------------------------------------ small instances
+
+// Instances small enough for one workgroup (64^2, 128^2): K steps in one
+// launch, one workgroup per instance, each step the same row and column
+// tiles as the two-kernel step (so the same results bit for bit) in turn,
+// separated by workgroup barriers -- no launch per pass, and the
+// instance's buffers stay in the L2 of its workgroup's XCD.
+template <int NC, int N, bool CONS, bool XDYN>
+struct FSmallCfg {
+    static constexpr int THREADS = 512;
+    static constexpr int LPB = THREADS / FRowCfg<NC, 1, 3>::T;
+    static constexpr int TWD = THREADS / FColCfg<N, 1>::T;
+    static constexpr int LDSN = LPB * FRowCfg<NC, LPB, 3>::RS > TWD * FColCfg<N, TWD>::LL
+                                    ? LPB * FRowCfg<NC, LPB, 3>::RS
+                                    : TWD * FColCfg<N, TWD>::LL;
+    static constexpr bool valid()
+    {
+        return FRowCfg<NC, LPB, 3>::valid() && FColCfg<N, TWD>::valid() && LPB <= N &&
+               (2 * NC + N + LDSN) * 8 <= 150 * 1024;
+    }
+};
+
+template <int NC, int N, bool CONS, bool XDYN>
+__global__ __launch_bounds__(512) void k_fsmall(const FusedInst *__restrict__ insts, const int *__restrict__ ids,
+                                                int n0, int k, int rel0, const float2 *__restrict__ sets,
+                                                unsigned long long *__restrict__ ts)
+{
+    using C = FSmallCfg<NC, N, CONS, XDYN>;
+    __shared__ float2 twr[2 * NC];
+    __shared__ float2 twc[N];
+    __shared__ float2 buf[C::LDSN];
+    unsigned long long *slot = ts ? ts + (size_t)(blockIdx.x % FUSED_TS_SLOTS) * FUSED_TS_STRIDE : nullptr;
+    if (slot && threadIdx.x == 0) atomicMin(&slot[0], (unsigned long long)wall_clock64());
+    const FusedInst &a = insts[ids[blockIdx.x]];
+    load_twiddles<2 * NC>(twr, sets);
+    load_twiddles<N>(twc, sets);
+    __syncthreads();
+    for (int n = n0; n < n0 + k; n++) {
+        const uint32_t rel = (uint32_t)(n - rel0);
+        for (int y0 = 0; y0 < N; y0 += C::LPB) frows_tile<NC, C::LPB, 3>(a, y0, rel, sets, false, twr, buf);
+        // rows ended with a barrier: X_g and eta are written, the buffer is free
+        for (int x0 = 0; x0 <= NC; x0 += C::TWD) fcols_tile<N, C::TWD, CONS, XDYN>(a, x0, n & 1, rel, 1, sets, false, twc, buf);
+        __syncthreads();  // Z_f and the spectra written; the buffer is free for the rows
+    }
+    if (slot && threadIdx.x == 0) atomicMax(&slot[1], (unsigned long long)wall_clock64());
 }
 
 // ------------------------------------------------------------ transposes
@@ -1370,6 +1441,41 @@ hipError_t fused_cols_launch(int ny, int twd, int stored, const FusedInst *inst,
     default:
         return hipErrorInvalidValue;
     }
+}
+
+bool fused_small_class(int nc, int ny, bool wide)
+{
+    return !wide && ny == 2 * nc && (ny == 64 || ny == 128);
+}
+
+template <int NC, int N>
+static hipError_t fsmall_n(int stored, const FusedInst *inst, const int *ids, int nids, int n0, int k, int rel0,
+                           hipStream_t s, const float2 *sets, unsigned long long *ts)
+{
+#define L(CONS, XDYN)                                                                                         \
+    static_assert(FSmallCfg<NC, N, CONS, XDYN>::valid(), "small-instance tile");                           \
+    hipLaunchKernelGGL((k_fsmall<NC, N, CONS, XDYN>), dim3(nids), dim3(FSmallCfg<NC, N, CONS, XDYN>::THREADS), \
+                       0, s, inst, ids, n0, k, rel0, sets, ts)
+    if (stored == 2) {
+        L(true, true);
+    } else if (stored == 1) {
+        L(true, false);
+    } else {
+        L(false, false);
+    }
+#undef L
+    return hipGetLastError();
+}
+
+hipError_t fused_small_launch(int nc, int ny, int stored, const FusedInst *inst, const int *ids, int nids, int n0,
+                              int k, int rel0, hipStream_t s, unsigned long long *ts)
+{
+    const float2 *sets = twiddle_sets(s);
+    if (!sets) return hipErrorOutOfMemory;
+    if (nids <= 0 || k <= 0) return hipSuccess;
+    if (nc == 32 && ny == 64) return fsmall_n<32, 64>(stored, inst, ids, nids, n0, k, rel0, s, sets, ts);
+    if (nc == 64 && ny == 128) return fsmall_n<64, 128>(stored, inst, ids, nids, n0, k, rel0, s, sets, ts);
+    return hipErrorInvalidValue;
 }
 
 hipError_t fused_transpose_in_launch(const FusedInst *inst, const FusedBlock *blocks, int nblocks, hipStream_t s)

@@ -161,6 +161,13 @@ hipError_t fused_rows_launch(int nc, int lpb, bool wide, const FusedInst *inst, 
 // constraints)
 hipError_t fused_cols_launch(int ny, int twd, int stored, const FusedInst *inst, const FusedBlock *blocks, int nblocks,
                              int parity, uint32_t step_rel, int mode, hipStream_t s, unsigned long long *ts = nullptr);
+// Size classes whose instances take whole steps in one workgroup each
+// (64^2, 128^2): k steps [n0, n0+k) of the instances ids[0..nids) in one
+// launch (step n reads spectrum buffer n&1, noise counter offset n - rel0);
+// stored and ts as for fused_cols_launch (one slot pair for the launch).
+bool fused_small_class(int nc, int ny, bool wide);
+hipError_t fused_small_launch(int nc, int ny, int stored, const FusedInst *inst, const int *ids, int nids, int n0,
+                              int k, int rel0, hipStream_t s, unsigned long long *ts = nullptr);
 // the slots of n timed launches to {~0, 0} (a kernel, so that graphs can hold it)
 hipError_t fused_ts_reset_launch(unsigned long long *ts, int n, hipStream_t s);
 // ---- 3D systems (one instance, natural half-spectrum layout; insts[0]).
