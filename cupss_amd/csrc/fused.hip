@@ -274,6 +274,22 @@ __global__ __launch_bounds__((FRowCfg<NC, LPB, MAXR>::THREADS)) void k_frows(con
 
 // ------------------------------------------------------------ column kernel
 
+// v[e] = p[col + t + e*T] for the E bins of a lane's column, or zeros for
+// an inactive column (one uniform branch for the column, the pointer read
+// once, constant offsets between the elements)
+template <int E, int T>
+__device__ __forceinline__ void load_col(float2 (&v)[E], const float2 *__restrict__ p, long col, int t, bool active)
+{
+    if (active) {
+        const float2 *__restrict__ q = p + col + t;
+#pragma unroll
+        for (int e = 0; e < E; e++) v[e] = gload(q, (long)e * T);
+    } else {
+#pragma unroll
+        for (int e = 0; e < E; e++) v[e] = make_float2(0.f, 0.f);
+    }
+}
+
 // Sum of the terms of one target field over the E bins of a lane: product
 // groups (FFT_y of X_g times the group multiplier) and single-field or
 // constant terms.  xv holds X of the first group on entry when `pre`.
@@ -287,10 +303,7 @@ __device__ __forceinline__ void target_sum(const FusedInst &a, const FusedTarget
 #pragma unroll
     for (int e = 0; e < E; e++) acc[e] = make_float2(0.f, 0.f);
     for (int gi = TG.g0; gi < TG.g1; gi++) {
-        if (!(pre && gi == TG.g0)) {
-#pragma unroll
-            for (int e = 0; e < E; e++) xv[e] = active ? gload(a.X[gi], col + t + e * T) : make_float2(0.f, 0.f);
-        }
+        if (!(pre && gi == TG.g0)) load_col<E, T>(xv, a.X[gi], col, t, active);
         fft_line<N, E, -1, N, !SKIP, (T <= 64)>(xv, L, t, tws);
         const TermRefDev &GM = a.gmul[gi];
         multiplier_col<E>(a.pool, GM.pres0, GM.npres, GM.mult_i, cb, m);
@@ -318,8 +331,7 @@ __device__ __forceinline__ void target_sum(const FusedInst &a, const FusedTarget
                 for (int e = 0; e < E; e++) X[e] = old[e];
             } else {
                 const float2 *__restrict__ src = a.cs[b][parity];
-#pragma unroll
-                for (int e = 0; e < E; e++) X[e] = active ? gload(src, col + t + e * T) : make_float2(0.f, 0.f);
+                load_col<E, T>(X, src, col, t, active);
             }
 #pragma unroll
             for (int e = 0; e < E; e++) X[e] = apply_mult(dm[e], DM.mult_i, X[e]);
@@ -327,8 +339,7 @@ __device__ __forceinline__ void target_sum(const FusedInst &a, const FusedTarget
             // SRC_DYN_NEW: a stored dynamic field's value after the step, which
             // this lane wrote to the other buffer earlier in the same pass
             const float2 *__restrict__ src = a.cs[TR.idx][TR.src == SRC_DYN_NEW ? parity ^ 1 : parity];
-#pragma unroll
-            for (int e = 0; e < E; e++) X[e] = active ? gload(src, col + t + e * T) : make_float2(0.f, 0.f);
+            load_col<E, T>(X, src, col, t, active);
         }
 #pragma unroll
         for (int e = 0; e < E; e++) acc[e] = cadd(acc[e], apply_mult(m[e], TR.mult_i, X[e]));
@@ -348,8 +359,9 @@ __device__ __forceinline__ void z_out(const FusedInst &a, int f, const Geom &g, 
         v[e] = (zero || !keep_mode2(g, a.kmax[f], a.kymax[f], cb.kx, t + e * T)) ? make_float2(0.f, 0.f) : val[e];
     fft_line<N, E, +1, N, !SKIP, (T <= 64)>(v, L, t, tws);
     if (active && cb.kx <= a.kmax[f]) {
+        float2 *__restrict__ z = a.Z[f] + col + t;  // read once: the stores could alias the descriptor
 #pragma unroll
-        for (int e = 0; e < E; e++) gstore(a.Z[f], col + t + e * T, v[e]);
+        for (int e = 0; e < E; e++) gstore(z, (long)e * T, v[e]);
     }
     line_sync<(T <= 64)>();
 }
@@ -394,8 +406,7 @@ __device__ __forceinline__ void stored_noise(const FusedInst &a, const FusedTarg
 #pragma unroll
     for (int e = 0; e < E; e++) ue[e] = make_float2(0.f, 0.f);
     if (TG.noisy == NOISE_REAL) {
-#pragma unroll
-        for (int e = 0; e < E; e++) ue[e] = active ? gload(TG.eta, col + t + e * T) : make_float2(0.f, 0.f);
+        load_col<E, T>(ue, TG.eta, col, t, active);
         fft_line<N, E, -1, N, !SKIP, (T <= 64)>(ue, L, t, tws);
         line_sync<(T <= 64)>();
     } else {
@@ -536,12 +547,8 @@ __device__ __forceinline__ void fcols_tile(const FusedInst &a, int line, int par
     const float2 *__restrict__ sin = a.spec[parity];
 
     float2 cur[E], xv[E];
-#pragma unroll
-    for (int e = 0; e < E; e++) {
-        const long i = col + t + e * T;
-        cur[e] = active ? gload(sin, i) : make_float2(0.f, 0.f);
-        xv[e] = (active && pre) ? gload(a.X[a.tgt[0].g0], i) : make_float2(0.f, 0.f);
-    }
+    load_col<E, T>(cur, sin, col, t, active);
+    load_col<E, T>(xv, pre ? a.X[a.tgt[0].g0] : sin, col, t, active && pre);
     if (load_tw) {
         load_twiddles<N>(tws, sets);
         __syncthreads();
@@ -582,8 +589,7 @@ __device__ __forceinline__ void fcols_tile(const FusedInst &a, int line, int par
         if (a.noise_mode == NOISE_REAL) {
             // loaded here rather than with phi: fewer registers live across
             // the transforms of the terms
-#pragma unroll
-            for (int e = 0; e < E; e++) eta[e] = active ? gload(a.eta, col + t + e * T) : make_float2(0.f, 0.f);
+            load_col<E, T>(eta, a.eta, col, t, active);
             fft_line<N, E, -1, N, !CONS, (T <= 64)>(eta, L, t, tws);
             line_sync<(T <= 64)>();
         } else if (a.noise_mode == NOISE_SPECTRAL) {
@@ -627,12 +633,11 @@ __device__ __forceinline__ void fcols_tile(const FusedInst &a, int line, int par
             float2 v[E];
             target_sum<N, E, T, !CONS>(a, TG, cb, cur, nw, xv, false, parity, active, col, L, t, tws, v);
             if (XDYN && TG.dyn) {
-                const float2 *__restrict__ uin = a.cs[ci][parity];
+                float2 u0[E];
+                load_col<E, T>(u0, a.cs[ci][parity], col, t, active);
+                const float dt = a.u.dt;
 #pragma unroll
-                for (int e = 0; e < E; e++) {
-                    const float2 u0 = active ? gload(uin, col + t + e * T) : make_float2(0.f, 0.f);
-                    v[e] = make_float2(u0.x + a.u.dt * v[e].x, u0.y + a.u.dt * v[e].y);
-                }
+                for (int e = 0; e < E; e++) v[e] = make_float2(u0[e].x + dt * v[e].x, u0[e].y + dt * v[e].y);
                 if (TG.noisy) stored_noise<N, E, T, !CONS>(a, TG, cb, kx, step, active, col, L, t, tws, v);
                 if (TG.nimp > 0) {
                     float rD[E];
@@ -647,8 +652,7 @@ __device__ __forceinline__ void fcols_tile(const FusedInst &a, int line, int par
                 // no terms: setNotDynamic_k leaves the value as it is, then
                 // divides it by the implicit operator's sum (if any)
                 const float2 *__restrict__ kin = a.cs[ci][parity];
-#pragma unroll
-                for (int e = 0; e < E; e++) v[e] = active ? gload(kin, col + t + e * T) : make_float2(0.f, 0.f);
+                load_col<E, T>(v, kin, col, t, active);
                 constraint_solve<E>(a, TG, cb, v);
             } else {
                 if (XDYN && TG.noisy) stored_noise<N, E, T, !CONS>(a, TG, cb, kx, step, active, col, L, t, tws, v);
@@ -675,8 +679,7 @@ __device__ __forceinline__ void fcols_tile(const FusedInst &a, int line, int par
             const bool dyn = XDYN && a.tgt[1 + ci].dyn != 0;
             if (z < 0 && !dyn) continue;
             float2 v[E];
-#pragma unroll
-            for (int e = 0; e < E; e++) v[e] = active ? gload(a.cs[ci][0], col + t + e * T) : make_float2(0.f, 0.f);
+            load_col<E, T>(v, a.cs[ci][0], col, t, active);
             if (z >= 0) z_out<N, E, T, !CONS>(a, z, g, cb, v, a.zero[z] != 0, blk.line, active, col, L, t, tws);
             if (dyn)
                 for (int f = 0; f < a.nreal; f++)
