@@ -61,7 +61,7 @@ struct FusedBatch {
     struct Class {
         int nc = 0, ny = 0;
         bool real = false;      // any instance has real fields (prologue)
-        bool wide = false;      // row kernel for more than 3 real fields
+        int width = 3;          // real fields the row kernel holds at once (fused_row_width)
         int cons = 0;           // stored fields of the instances: 1 = constraints, 2 = dynamic fields too
         std::vector<int> insts;
         int lpb = 0, twd = 0;   // tile shapes
@@ -511,7 +511,7 @@ static int class_tables(const FusedBatch &B, const FusedBatch::Class &c, int lpb
 // 256 threads per column workgroup
 static void default_shapes(FusedBatch::Class &c)
 {
-    const std::vector<int> rs = fused_row_shapes(c.nc, c.ny, c.wide), cs = fused_col_shapes(c.ny);
+    const std::vector<int> rs = fused_row_shapes(c.nc, c.ny, c.width), cs = fused_col_shapes(c.ny);
     c.lpb = rs.empty() ? 0 : rs.front();
     for (int v : rs)
         if (v <= 8) c.lpb = v;
@@ -553,13 +553,14 @@ static int build(FusedBatch &B)
         int stored = a.ncons > 0 ? 1 : 0;
         for (int k = 0; k < a.ncons; k++)
             if (a.tgt[1 + k].dyn || a.tgt[1 + k].fresh || a.tgt[1 + k].noisy) stored = 2;
-        auto key = std::make_pair(a.g.nc * 8 + (a.nreal > 3 ? 4 : 0) + stored, a.g.sy);
+        const int width = fused_row_width(a.nreal);
+        auto key = std::make_pair(a.g.nc * 16 + (width == 2 ? 0 : width == 3 ? 4 : 8) + stored, a.g.sy);
         if (!cls.count(key)) {
             cls[key] = (int)B.classes.size();
             B.classes.emplace_back();
             B.classes.back().nc = a.g.nc;
             B.classes.back().ny = a.g.sy;
-            B.classes.back().wide = a.nreal > 3;
+            B.classes.back().width = width;
             B.classes.back().cons = stored;
         }
         FusedBatch::Class &c = B.classes[cls[key]];
@@ -572,7 +573,7 @@ static int build(FusedBatch &B)
         default_shapes(c);
         if (c.lpb <= 0 || c.twd <= 0) return fail(CUPSS_ERR_UNSUPPORTED, "no tile shape for this size class");
         if (int r = class_tables(B, c, c.lpb, c.twd, B.chunk_bytes, c.chunks)) return r;
-        c.small = fused_small_class(c.nc, c.ny, c.wide);
+        c.small = fused_small_class(c.nc, c.ny, c.width);
         if (c.small) {
             if (hipMalloc(&c.ids, sizeof(int) * c.insts.size()) != hipSuccess)
                 return fail(CUPSS_ERR_NOMEM, "device allocation failed");
@@ -678,7 +679,7 @@ static int launch_class(FusedBatch &B, size_t ci, int n0, int k, int rel0, hipSt
     }
     for (auto &t : c.chunks)
         for (int n = n0; n < n0 + k; n++) {
-            HIP_TRY(fused_rows_launch(c.nc, c.lpb, c.wide, B.d, t.rb, t.nrb, (uint32_t)(n - rel0), st));
+            HIP_TRY(fused_rows_launch(c.nc, c.lpb, c.width, B.d, t.rb, t.nrb, (uint32_t)(n - rel0), st));
             unsigned long long *ts = nullptr;
             if (timed && B.pcount < FusedBatch::PROF_SLOTS)
                 ts = B.pts + (size_t)(B.pcount++) * FUSED_TS_SLOTS * FUSED_TS_STRIDE;
@@ -826,14 +827,14 @@ static int tune(FusedBatch &B)
         // rows
         float best = 1e30f;
         int pick = c.lpb;
-        for (int lpb : fused_row_shapes(c.nc, c.ny, c.wide)) {
+        for (int lpb : fused_row_shapes(c.nc, c.ny, c.width)) {
             std::vector<Tables> t;
             if (int r = class_tables(B, c, lpb, c.twd, 0, t)) return r;
             float ms = 0;
             const int r = time_ms(
                 B, T, reps,
                 [&]() -> int {
-                    HIP_TRY(fused_rows_launch(c.nc, lpb, c.wide, B.d, t[0].rb, t[0].nrb, 0, B.stream));
+                    HIP_TRY(fused_rows_launch(c.nc, lpb, c.width, B.d, t[0].rb, t[0].nrb, 0, B.stream));
                     return CUPSS_OK;
                 },
                 &ms);
@@ -1252,7 +1253,7 @@ static std::string schedule_text(const FusedBatch &B)
     char line[256];
     for (const auto &c : B.classes) {
         std::snprintf(line, sizeof line, "class %dx%d%s%s: %zu instances, rows/wg %d, cols/wg %d, chunks %zu\n",
-                      2 * c.nc, c.ny, c.wide ? " (wide)" : "", c.cons ? " (stored fields)" : "", c.insts.size(),
+                      2 * c.nc, c.ny, c.width > 3 ? " (wide)" : "", c.cons ? " (stored fields)" : "", c.insts.size(),
                       c.lpb, c.twd, c.chunks.size());
         out += line;
     }

@@ -172,6 +172,13 @@ __host__ __device__ constexpr int row_elems(int nc)
 __host__ __device__ constexpr long xcd_blocks(long total) { return (total + 7) / 8 * 8; }
 __device__ __forceinline__ long xcd_item(long b, long nblk) { return (b & 7) * (nblk >> 3) + (b >> 3); }
 
+// a table entry exp(-2*pi*i*x) as exp(DIR*2*pi*i*x)
+template <int DIR>
+__device__ __forceinline__ float2 tw_dir(float2 w)
+{
+    return DIR > 0 ? make_float2(w.x, -w.y) : w;
+}
+
 // exp(DIR*2*pi*i*m/L) from a table of exp(-2*pi*i*j/TWS), j < TWS
 // (L | TWS, 0 <= m).  The table lives in global memory (TWS = TW_N) or in
 // the workgroup's LDS (TWS = the largest length the kernel transforms).
@@ -189,6 +196,131 @@ template <int M>
 __device__ __forceinline__ void load_twiddles(float2 *__restrict__ dst, const float2 *__restrict__ sets)
 {
     for (int i = threadIdx.x; i < M; i += blockDim.x) dst[i] = sets[M + i];
+}
+
+// The same copy (table M, 16-byte aligned in LDS) by a workgroup of TH
+// threads in two halves, as StageTwiddles below: all of a thread's loads
+// are in flight at once (up to 2 each; more fall back to a loop in store()).
+template <int M, int TH>
+struct SizeTwiddles {
+    static constexpr int M4 = M / 2;
+    static constexpr int PER = M4 > 0 ? (M4 + TH - 1) / TH : 1;
+    static constexpr bool REG = M % 2 == 0 && PER <= 2;
+    float4 r[REG ? PER : 1];
+    __device__ __forceinline__ void load(const float2 *__restrict__ sets)
+    {
+        if constexpr (REG) {
+            const float4 *__restrict__ src = (const float4 *)(sets + M);
+#pragma unroll
+            for (int p = 0; p < PER; p++) {
+                const int i = threadIdx.x + p * TH;
+                if (i < M4) r[p] = gload(src, i);
+            }
+        }
+    }
+    __device__ __forceinline__ void store(float2 *__restrict__ dst, const float2 *__restrict__ sets) const
+    {
+        if constexpr (REG) {
+#pragma unroll
+            for (int p = 0; p < PER; p++) {
+                const int i = threadIdx.x + p * TH;
+                if (i < M4) ((float4 *)dst)[i] = r[p];
+            }
+        } else {
+            for (int i = threadIdx.x; i < M; i += TH) dst[i] = sets[M + i];
+        }
+    }
+};
+
+// ---- stage-ordered twiddles (TWS = TW_STAGE)
+//
+// For the transform of N points with E per lane, the stage of span NS > 1
+// and radix R needs w^(r*k) = exp(-2*pi*i*r*k/(NS*R)) for r = 1..R-1 and
+// k = j mod NS.  The stage table holds them in the order the lanes read
+// them: for R >= 4, entry 2*(q*NS + k) + p is w^((2q+p)*k), q < R/2, so a
+// lane fetches its R twiddles as R/2 aligned 16-byte pairs at one address
+// (k) plus constant offsets, and consecutive k are consecutive pairs (no
+// bank conflicts); for R = 2, entry k is w^k.  The stages' tables follow
+// each other in stage order.  The tables of every (N, E) sit in the same
+// device array as the per-size tables (twiddle_sets()), at tw_stage_offset.
+constexpr int TW_STAGE = 0;
+
+__host__ __device__ constexpr int tw_stage_radix(int N, int E, int NS)
+{
+    return N / NS >= E ? (E >= 2 ? E : 2) : N / NS;
+}
+__host__ __device__ constexpr int tw_stage_len(int N, int E, int NS)
+{
+    return tw_stage_radix(N, E, NS) >= 4 ? tw_stage_radix(N, E, NS) * NS : NS;
+}
+// offset of the table of the stage with span NS (its sum of the earlier
+// twiddled stages' lengths); tw_stage_base(N, E, N) is the whole size
+__host__ __device__ constexpr int tw_stage_base(int N, int E, int NS)
+{
+    int off = 0;
+    for (int ns = tw_stage_radix(N, E, 1); ns < NS && ns < N; ns *= tw_stage_radix(N, E, ns))
+        off += tw_stage_len(N, E, ns);
+    return off;
+}
+__host__ __device__ constexpr int tw_stage_size(int N, int E) { return tw_stage_base(N, E, N); }
+// LDS entries to declare for them (a one-stage transform has none)
+__host__ __device__ constexpr int tw_stage_alloc(int N, int E) { return tw_stage_size(N, E) > 0 ? tw_stage_size(N, E) : 2; }
+// Position of the (N, E) tables in the twiddle-set array: after the 2*TW_N
+// entries of the per-size tables, one region of 4*TW_N per E = 1, 2, 4, 8,
+// 16 holding the table of N at [2N, 4N) of the region (every table of N
+// points is shorter than 2N entries)
+__host__ __device__ constexpr long tw_stage_offset(int N, int E)
+{
+    return 2L * TW_N + (long)ilog2c(E) * 4 * TW_N + 2L * N;
+}
+constexpr long TW_SETS_LEN = 2L * TW_N + 5L * 4 * TW_N;
+
+// The stage tables of (N, E), copied into LDS (16-byte aligned) by a
+// workgroup of TH threads in two halves: load() issues the global loads
+// into registers, store() writes them to LDS.  A kernel issues its own
+// loads between the two, so that waiting for the table does not also wait
+// for them (loads complete in order for the wait counter).
+template <int N, int E, int TH>
+struct StageTwiddles {
+    static constexpr int M4 = tw_stage_size(N, E) / 2;  // 16-byte pairs
+    static constexpr int PER = M4 > 0 ? (M4 + TH - 1) / TH : 1;
+    static constexpr bool REG = PER <= 2;  // more: a loop in store() (registers)
+    static_assert(tw_stage_size(N, E) % 2 == 0, "stage tables hold whole pairs");
+    float4 r[REG ? PER : 1];
+    __device__ __forceinline__ void load(const float2 *__restrict__ sets)
+    {
+        if constexpr (REG) {
+            const float4 *__restrict__ src = (const float4 *)(sets + tw_stage_offset(N, E));
+#pragma unroll
+            for (int p = 0; p < PER; p++) {
+                const int i = threadIdx.x + p * TH;
+                if (i < M4) r[p] = gload(src, i);
+            }
+        }
+    }
+    __device__ __forceinline__ void store(float2 *__restrict__ dst, const float2 *__restrict__ sets) const
+    {
+        if constexpr (REG) {
+#pragma unroll
+            for (int p = 0; p < PER; p++) {
+                const int i = threadIdx.x + p * TH;
+                if (i < M4) ((float4 *)dst)[i] = r[p];
+            }
+        } else {
+            const float4 *__restrict__ src = (const float4 *)(sets + tw_stage_offset(N, E));
+            for (int i = threadIdx.x; i < M4; i += TH) ((float4 *)dst)[i] = gload(src, i);
+        }
+    }
+};
+
+// the same in one call (any workgroup size)
+template <int N, int E>
+__device__ __forceinline__ void load_stage_twiddles(float2 *__restrict__ dst, const float2 *__restrict__ sets)
+{
+    constexpr int M4 = tw_stage_size(N, E) / 2;
+    const float4 *__restrict__ src = (const float4 *)(sets + tw_stage_offset(N, E));
+    float4 *__restrict__ d = (float4 *)dst;
+    for (int i = threadIdx.x; i < M4; i += blockDim.x) d[i] = gload(src, i);
 }
 
 // Synchronisation of one line's LDS exchanges.  WL (wave-local): the line's
@@ -231,7 +363,19 @@ struct Stockham {
                 for (int r = 0; r < R; r++) a[r] = v[b + r * (E / R)];
                 const int j = t + b * T;
                 const int k = j & (NS - 1);
-                if constexpr (NS > 1) {
+                if constexpr (NS > 1 && TWS == TW_STAGE) {
+                    const float2 *__restrict__ st = tw + tw_stage_base(N, E, NS);
+                    if constexpr (R >= 4) {
+#pragma unroll
+                        for (int q = 0; q < R / 2; q++) {
+                            const float4 w = ((const float4 *)st)[q * NS + k];
+                            if (q > 0) a[2 * q] = cmul(a[2 * q], tw_dir<DIR>(make_float2(w.x, w.y)));
+                            a[2 * q + 1] = cmul(a[2 * q + 1], tw_dir<DIR>(make_float2(w.z, w.w)));
+                        }
+                    } else {
+                        a[1] = cmul(a[1], tw_dir<DIR>(st[k]));
+                    }
+                } else if constexpr (NS > 1) {
 #pragma unroll
                     for (int r = 1; r < R; r++) a[r] = cmul(a[r], tw_lookup<DIR, TWS>(tw, r * k, NS * R));
                 }

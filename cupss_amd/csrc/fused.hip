@@ -45,7 +45,7 @@ namespace cupss {
 
 // ------------------------------------------------------------ row kernel
 
-// MAXR: real fields a row holds at once (3, or FUSED_MAX_REAL for systems
+// MAXR: real fields a row holds at once (2, 3, or FUSED_MAX_REAL for systems
 // with more; those use 4 elements per lane to stay within the registers).
 // Rows of 1024 real points also take 4 per lane (more waves for one large
 // system).
@@ -57,7 +57,7 @@ struct FRowCfg {
     static constexpr int RS = lds_len(NC);  // tile row stride (>= NC + 1), odd for NC >= 32
     static constexpr bool valid()
     {
-        return THREADS >= 16 && THREADS <= 512 && (LPB * RS + 2 * NC) * 8 <= 150 * 1024;
+        return THREADS >= 16 && THREADS <= 512 && (LPB * RS + 2 * NC + tw_stage_size(NC, E)) * 8 <= 150 * 1024;
     }
 };
 
@@ -67,24 +67,87 @@ typedef float vf4 __attribute__((ext_vector_type(4)));
 
 // rows y0 .. y0+LPB-1 of a column-major half spectrum into the tile (row l
 // at l*RS, entry k at k); columns k > kmax read as zero.  Pairs of rows are
-// moved with one 16-byte access.
+// moved with one 16-byte access.  issue() starts a thread's loads (all at
+// once where it has at most CH), commit() writes them to the tile (and does
+// the whole copy in rounds of CH loads where there are more), so that the
+// loads are in flight together instead of one round trip each.
+template <int NC, int LPB, int RS, int TH, int CH = 4>
+struct TileLoad {
+    static constexpr int HP = LPB / 2;
+    static constexpr int ITEMS = (NC + 1) * (LPB >= 2 ? HP : 1);
+    static constexpr int IT = (ITEMS + TH - 1) / TH;
+    static constexpr bool ONE = IT <= CH;
+    vf4 w[ONE ? IT : 1];
+    __device__ __forceinline__ vf4 item(const float2 *__restrict__ src, int sy, int y0, int kmax, int i) const
+    {
+        vf4 v = {0.f, 0.f, 0.f, 0.f};
+        if constexpr (LPB >= 2) {
+            const int k = i / HP, yl = 2 * (i % HP);
+            if (k <= kmax) v = gload(reinterpret_cast<const vf4 *>(src + (long)k * sy + y0 + yl), 0);
+        } else {
+            if (i <= kmax) {
+                const float2 z = gload(src, (long)i * sy + y0);
+                v.x = z.x;
+                v.y = z.y;
+            }
+        }
+        return v;
+    }
+    __device__ __forceinline__ void put(float2 *__restrict__ tile, int i, vf4 v) const
+    {
+        if constexpr (LPB >= 2) {
+            const int k = i / HP, yl = 2 * (i % HP);
+            tile[yl * RS + k] = make_float2(v.x, v.y);
+            tile[(yl + 1) * RS + k] = make_float2(v.z, v.w);
+        } else {
+            tile[i] = make_float2(v.x, v.y);
+        }
+    }
+    __device__ __forceinline__ void issue(const float2 *__restrict__ src, int sy, int y0, int kmax)
+    {
+        if constexpr (ONE) {
+#pragma unroll
+            for (int p = 0; p < IT; p++) {
+                const int i = threadIdx.x + p * TH;
+                if (i < ITEMS) w[p] = item(src, sy, y0, kmax, i);
+            }
+        }
+    }
+    __device__ __forceinline__ void commit(float2 *__restrict__ tile, const float2 *__restrict__ src, int sy, int y0,
+                                           int kmax)
+    {
+        if constexpr (ONE) {
+#pragma unroll
+            for (int p = 0; p < IT; p++) {
+                const int i = threadIdx.x + p * TH;
+                if (i < ITEMS) put(tile, i, w[p]);
+            }
+        } else {
+#pragma unroll
+            for (int p0 = 0; p0 < IT; p0 += CH) {
+                vf4 v[CH];
+#pragma unroll
+                for (int q = 0; q < CH; q++) {
+                    const int i = threadIdx.x + (p0 + q) * TH;
+                    if (p0 + q < IT && i < ITEMS) v[q] = item(src, sy, y0, kmax, i);
+                }
+#pragma unroll
+                for (int q = 0; q < CH; q++) {
+                    const int i = threadIdx.x + (p0 + q) * TH;
+                    if (p0 + q < IT && i < ITEMS) put(tile, i, v[q]);
+                }
+            }
+        }
+    }
+};
+
 template <int NC, int LPB, int RS, int TH>
 __device__ __forceinline__ void tile_load(float2 *__restrict__ tile, const float2 *__restrict__ src, int sy, int y0,
                                           int kmax)
 {
-    if constexpr (LPB >= 2) {
-        constexpr int HP = LPB / 2;
-        for (int i = threadIdx.x; i < (NC + 1) * HP; i += TH) {
-            const int k = i / HP, yl = 2 * (i % HP);
-            vf4 w = {0.f, 0.f, 0.f, 0.f};
-            if (k <= kmax) w = gload(reinterpret_cast<const vf4 *>(src + (long)k * sy + y0 + yl), 0);
-            tile[yl * RS + k] = make_float2(w.x, w.y);
-            tile[(yl + 1) * RS + k] = make_float2(w.z, w.w);
-        }
-    } else {
-        for (int k = threadIdx.x; k <= NC; k += TH)
-            tile[k] = k <= kmax ? gload(src, (long)k * sy + y0) : make_float2(0.f, 0.f);
-    }
+    TileLoad<NC, LPB, RS, TH> tl;
+    tl.issue(src, sy, y0, kmax);
+    tl.commit(tile, src, sy, y0, kmax);
 }
 
 template <int NC, int LPB, int RS, int TH>
@@ -126,10 +189,12 @@ __device__ __forceinline__ void c2r_prep(const float2 (&xa)[E], const float2 (&y
 
 // R2C of one row whose real values are packed as v[e] = (x[2m], x[2m+1]);
 // the NC+1 bins are left in L[0..NC] (unpadded).  Ends with a barrier.
-template <int NC, int E, int T>
-__device__ __forceinline__ void r2c_tile(float2 (&v)[E], float2 *L, int t, const float2 *tws)
+// tws: the 2NC-point table (post-twiddles); the NC-point transform takes
+// its twiddles from ftw (TWF = TW_STAGE: the stage tables, else tws).
+template <int NC, int E, int T, int TWF>
+__device__ __forceinline__ void r2c_tile(float2 (&v)[E], float2 *L, int t, const float2 *tws, const float2 *ftw)
 {
-    fft_line<NC, E, -1, 2 * NC, true, (T <= 64)>(v, L, t, tws);
+    fft_line<NC, E, -1, TWF, true, (T <= 64)>(v, L, t, ftw);
     float2 o[E];
 #pragma unroll
     for (int e = 0; e < E; e++) {
@@ -149,11 +214,13 @@ __device__ __forceinline__ void r2c_tile(float2 (&v)[E], float2 *L, int t, const
 }
 
 // Rows y0 .. y0+LPB-1 of one instance (the whole workgroup).  load_tw:
-// copy the twiddle table into tws first (otherwise it is there already).
+// copy the twiddle tables into tws (2NC points) and tst (the NC-point
+// transform's stage tables) first (otherwise they are there already).
 // Ends with a workgroup barrier: the tile and the rows written are free.
 template <int NC, int LPB, int MAXR>
 __device__ __forceinline__ void frows_tile(const FusedInst &a, int y0, uint32_t step_rel,
-                                           const float2 *__restrict__ sets, bool load_tw, float2 *tws, float2 *tile)
+                                           const float2 *__restrict__ sets, bool load_tw, float2 *tws, float2 *tst,
+                                           float2 *tile)
 {
     using C = FRowCfg<NC, LPB, MAXR>;
     constexpr int E = C::E, T = C::T, RS = C::RS, TH = C::THREADS;
@@ -162,13 +229,28 @@ __device__ __forceinline__ void frows_tile(const FusedInst &a, int y0, uint32_t 
     const int l = threadIdx.x / T, t = threadIdx.x % T;
     float2 *L = tile + l * RS;
     const int nreal = a.nreal;
-    if (load_tw) load_twiddles<2 * NC>(tws, sets);
+    // twiddle loads first, then the first tile's: their stores then wait
+    // for the twiddles only
+    SizeTwiddles<2 * NC, TH> twl;
+    StageTwiddles<NC, E, TH> tsl;
+    if (load_tw) {
+        twl.load(sets);
+        tsl.load(sets);
+    }
 
     float2 r[MAXR][E];
 #pragma unroll
     for (int f = 0; f < MAXR; f++) {
         if (f < nreal) {
-            tile_load<NC, LPB, RS, TH>(tile, a.Z[f], sy, y0, a.kmax[f]);
+            // wide rows hold up to FUSED_MAX_REAL fields in registers: one
+            // load in flight there
+            TileLoad<NC, LPB, RS, TH, (MAXR > 3 ? 1 : 4)> tl;
+            tl.issue(a.Z[f], sy, y0, a.kmax[f]);
+            if (f == 0 && load_tw) {
+                twl.store(tws, sets);
+                tsl.store(tst, sets);
+            }
+            tl.commit(tile, a.Z[f], sy, y0, a.kmax[f]);
             __syncthreads();
             float2 xa[E], ya[E], v[E];
 #pragma unroll
@@ -177,15 +259,23 @@ __device__ __forceinline__ void frows_tile(const FusedInst &a, int y0, uint32_t 
                 xa[e] = L[k];
                 ya[e] = L[NC - k];
             }
-            __syncthreads();  // the tile becomes the transform's LDS
+            // the row's part of the tile becomes its transform's LDS (the
+            // row's lanes are the only ones that read or write it from here)
+            line_sync<(T <= 64)>();
             c2r_prep<NC, E, T>(xa, ya, t, v, tws);
-            fft_line<NC, E, +1, 2 * NC, false, (T <= 64)>(v, L, t, tws);
+            fft_line<NC, E, +1, TW_STAGE, false, (T <= 64)>(v, L, t, tst);
 #pragma unroll
             for (int e = 0; e < E; e++) r[f][e] = cscale(v[e], a.scale);
             __syncthreads();
         }
     }
-    if (nreal == 0) __syncthreads();  // twiddles
+    if (nreal == 0) {
+        if (load_tw) {
+            twl.store(tws, sets);
+            tsl.store(tst, sets);
+        }
+        __syncthreads();  // twiddles
+    }
     for (int gi = 0; gi < a.ngroups; gi++) {
         float2 v[E];
 #pragma unroll
@@ -209,7 +299,7 @@ __device__ __forceinline__ void frows_tile(const FusedInst &a, int y0, uint32_t 
 #pragma unroll
             for (int e = 0; e < E; e++) v[e] = cadd(v[e], p[e]);
         }
-        r2c_tile<NC, E, T>(v, L, t, tws);
+        r2c_tile<NC, E, T, TW_STAGE>(v, L, t, tws, tst);
         tile_store<NC, LPB, RS, TH>(a.X[gi], tile, sy, y0);
         __syncthreads();
     }
@@ -252,7 +342,7 @@ __device__ __forceinline__ void frows_tile(const FusedInst &a, int y0, uint32_t 
             v[e2] = even ? own : rcv;
             v[e2 + E / 2] = even ? rcv : own;
         }
-        r2c_tile<NC, E, T>(v, L, t, tws);
+        r2c_tile<NC, E, T, TW_STAGE>(v, L, t, tws, tst);
         tile_store<NC, LPB, RS, TH>(eta, tile, sy, y0);
         __syncthreads();  // the tile is the next source's transform LDS
     }
@@ -265,11 +355,12 @@ __global__ __launch_bounds__((FRowCfg<NC, LPB, MAXR>::THREADS)) void k_frows(con
                                                                           const float2 *__restrict__ sets)
 {
     using C = FRowCfg<NC, LPB, MAXR>;
-    __shared__ float2 tws[2 * NC];
+    __shared__ __align__(16) float2 tws[2 * NC];
+    __shared__ __align__(16) float2 tst[tw_stage_alloc(NC, C::E)];
     __shared__ float2 tile[LPB * C::RS];
     const FusedBlock blk = blocks[blockIdx.x];
     if (blk.inst < 0) return;
-    frows_tile<NC, LPB, MAXR>(insts[blk.inst], blk.line, step_rel, sets, true, tws, tile);
+    frows_tile<NC, LPB, MAXR>(insts[blk.inst], blk.line, step_rel, sets, true, tws, tst, tile);
 }
 
 // ------------------------------------------------------------ column kernel
@@ -304,7 +395,7 @@ __device__ __forceinline__ void target_sum(const FusedInst &a, const FusedTarget
     for (int e = 0; e < E; e++) acc[e] = make_float2(0.f, 0.f);
     for (int gi = TG.g0; gi < TG.g1; gi++) {
         if (!(pre && gi == TG.g0)) load_col<E, T>(xv, a.X[gi], col, t, active);
-        fft_line<N, E, -1, N, !SKIP, (T <= 64)>(xv, L, t, tws);
+        fft_line<N, E, -1, TW_STAGE, !SKIP, (T <= 64)>(xv, L, t, tws);
         const TermRefDev &GM = a.gmul[gi];
         multiplier_col<E>(a.pool, GM.pres0, GM.npres, GM.mult_i, cb, m);
 #pragma unroll
@@ -357,7 +448,7 @@ __device__ __forceinline__ void z_out(const FusedInst &a, int f, const Geom &g, 
 #pragma unroll
     for (int e = 0; e < E; e++)
         v[e] = (zero || !keep_mode2(g, a.kmax[f], a.kymax[f], cb.kx, t + e * T)) ? make_float2(0.f, 0.f) : val[e];
-    fft_line<N, E, +1, N, !SKIP, (T <= 64)>(v, L, t, tws);
+    fft_line<N, E, +1, TW_STAGE, !SKIP, (T <= 64)>(v, L, t, tws);
     if (active && cb.kx <= a.kmax[f]) {
         float2 *__restrict__ z = a.Z[f] + col + t;  // read once: the stores could alias the descriptor
 #pragma unroll
@@ -407,7 +498,7 @@ __device__ __forceinline__ void stored_noise(const FusedInst &a, const FusedTarg
     for (int e = 0; e < E; e++) ue[e] = make_float2(0.f, 0.f);
     if (TG.noisy == NOISE_REAL) {
         load_col<E, T>(ue, TG.eta, col, t, active);
-        fft_line<N, E, -1, N, !SKIP, (T <= 64)>(ue, L, t, tws);
+        fft_line<N, E, -1, TW_STAGE, !SKIP, (T <= 64)>(ue, L, t, tws);
         line_sync<(T <= 64)>();
     } else {
         const uint64_t s = step + (uint64_t)TG.step_off;
@@ -546,22 +637,46 @@ __device__ __forceinline__ void fcols_tile(const FusedInst &a, int line, int par
     const bool pre = upd && a.tgt[0].g1 > a.tgt[0].g0 && !(XDYN && a.nfresh > 0);
     const float2 *__restrict__ sin = a.spec[parity];
 
+    // the twiddle loads first: waiting for them then leaves the state's
+    // loads in flight
+    StageTwiddles<N, E, C::THREADS> twr;
+    if (load_tw) twr.load(sets);
     float2 cur[E], xv[E];
     load_col<E, T>(cur, sin, col, t, active);
     load_col<E, T>(xv, pre ? a.X[a.tgt[0].g0] : sin, col, t, active && pre);
-    if (load_tw) {
-        load_twiddles<N>(tws, sets);
-        __syncthreads();
-    }
 
     ColBins<E> cb;
     col_bins<E, T>(g, kx, t, cb);
+    const uint64_t step = a.step_base + step_rel;
+    // phi's spectral noise, drawn while the state's loads are in flight and
+    // added to phi_m at once (no register block held across the transforms)
+    // where nothing else reads phi_m: no single-field terms, no stored
+    // fields.  phi_{m+1} = (phi_m + pn eta + dt acc) / D then.  ky and
+    // ky + N/2 are elements e and e + E/2 of this lane.
+    const bool early = upd && a.noise_mode == NOISE_SPECTRAL && a.u.noisy && a.tgt[0].nterms == 0 &&
+                       (!CONS || a.ncons == 0);
+    if (early) {
+        float2 eta[E];
+#pragma unroll
+        for (int e = 0; e < E / 2; e++)
+            spectral_noise_pair(g, kx, t + e * T, 0, (uint32_t)step, (uint32_t)(step >> 32), a.stream, a.key0,
+                                a.key1, eta[e], eta[e + E / 2]);
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const float pn = noise_amplitude_col<E>(a.u, cb, e);
+            cur[e].x += pn * eta[e].x;
+            cur[e].y += pn * eta[e].y;
+        }
+    }
+    if (load_tw) {
+        twr.store(tws, sets);
+        __syncthreads();
+    }
     float2 nw[E];  // phi_{m+1}
     // workgroups holding the column kx = 0 or kx = sx/2 (uniform test)
     const bool edge_wg = blk.line == 0 || blk.line + TWD > g.nc;
     const bool edge = active && (kx == 0 || kx == g.nc);
     if (upd) {
-        const uint64_t step = a.step_base + step_rel;
         // ---- fresh constraints (setNotDynamic_k of phase A, src/field_kernels.cu:
         // 130-197): their values of step m, from the dynamic fields of step m,
         // into the buffer the dynamic fields' terms read
@@ -590,10 +705,9 @@ __device__ __forceinline__ void fcols_tile(const FusedInst &a, int line, int par
             // loaded here rather than with phi: fewer registers live across
             // the transforms of the terms
             load_col<E, T>(eta, a.eta, col, t, active);
-            fft_line<N, E, -1, N, !CONS, (T <= 64)>(eta, L, t, tws);
+            fft_line<N, E, -1, TW_STAGE, !CONS, (T <= 64)>(eta, L, t, tws);
             line_sync<(T <= 64)>();
-        } else if (a.noise_mode == NOISE_SPECTRAL) {
-            // ky and ky + N/2 are elements e and e + E/2 of this lane
+        } else if (a.noise_mode == NOISE_SPECTRAL && !early) {
 #pragma unroll
             for (int e = 0; e < E / 2; e++)
                 spectral_noise_pair(g, kx, t + e * T, 0, (uint32_t)step, (uint32_t)(step >> 32), a.stream, a.key0,
@@ -606,7 +720,7 @@ __device__ __forceinline__ void fcols_tile(const FusedInst &a, int line, int par
             float2 s = cur[e];
             s.x += a.u.dt * acc[e].x;
             s.y += a.u.dt * acc[e].y;
-            if (a.u.noisy) {
+            if (a.u.noisy && !early) {
                 const float pn = noise_amplitude_col<E>(a.u, cb, e);
                 s.x += pn * eta[e].x;
                 s.y += pn * eta[e].y;
@@ -702,7 +816,7 @@ __global__ __launch_bounds__((FColCfg<N, TWD>::THREADS)) void k_fcols(const Fuse
                                                                    unsigned long long *__restrict__ ts)
 {
     using C = FColCfg<N, TWD>;
-    __shared__ float2 tws[N];
+    __shared__ __align__(16) float2 tws[tw_stage_alloc(N, C::E)];
     __shared__ float2 lds[TWD * C::LL];
     const FusedBlock blk = blocks[blockIdx.x];
     if (blk.inst < 0) return;
@@ -737,7 +851,9 @@ struct FSmallCfg {
     static constexpr bool valid()
     {
         return FRowCfg<NC, LPB, 3>::valid() && FColCfg<N, TWD>::valid() && LPB <= N &&
-               (2 * NC + N + LDSN) * 8 <= 150 * 1024;
+               (2 * NC + tw_stage_size(NC, FRowCfg<NC, LPB, 3>::E) + tw_stage_size(N, FColCfg<N, TWD>::E) + LDSN) *
+                       8 <=
+                   150 * 1024;
     }
 };
 
@@ -748,17 +864,19 @@ __global__ __launch_bounds__(512) void k_fsmall(const FusedInst *__restrict__ in
 {
     using C = FSmallCfg<NC, N, CONS, XDYN>;
     __shared__ float2 twr[2 * NC];
-    __shared__ float2 twc[N];
+    __shared__ __align__(16) float2 tsr[tw_stage_alloc(NC, FRowCfg<NC, C::LPB, 3>::E)];
+    __shared__ __align__(16) float2 twc[tw_stage_alloc(N, FColCfg<N, C::TWD>::E)];
     __shared__ float2 buf[C::LDSN];
     unsigned long long *slot = ts ? ts + (size_t)(blockIdx.x % FUSED_TS_SLOTS) * FUSED_TS_STRIDE : nullptr;
     if (slot && threadIdx.x == 0) atomicMin(&slot[0], (unsigned long long)wall_clock64());
     const FusedInst &a = insts[ids[blockIdx.x]];
     load_twiddles<2 * NC>(twr, sets);
-    load_twiddles<N>(twc, sets);
+    load_stage_twiddles<NC, FRowCfg<NC, C::LPB, 3>::E>(tsr, sets);
+    load_stage_twiddles<N, FColCfg<N, C::TWD>::E>(twc, sets);
     __syncthreads();
     for (int n = n0; n < n0 + k; n++) {
         const uint32_t rel = (uint32_t)(n - rel0);
-        for (int y0 = 0; y0 < N; y0 += C::LPB) frows_tile<NC, C::LPB, 3>(a, y0, rel, sets, false, twr, buf);
+        for (int y0 = 0; y0 < N; y0 += C::LPB) frows_tile<NC, C::LPB, 3>(a, y0, rel, sets, false, twr, tsr, buf);
         // rows ended with a barrier: X_g and eta are written, the buffer is free
         for (int x0 = 0; x0 <= NC; x0 += C::TWD) fcols_tile<N, C::TWD, CONS, XDYN>(a, x0, n & 1, rel, 1, sets, false, twc, buf);
         __syncthreads();  // Z_f and the spectra written; the buffer is free for the rows
@@ -1040,7 +1158,7 @@ __global__ __launch_bounds__((FRowCfg<NC, LPB, MAXR>::THREADS)) void k_f3rows(co
 #pragma unroll
             for (int e = 0; e < E; e++) v[e] = cadd(v[e], p[e]);
         }
-        r2c_tile<NC, E, T>(v, L, t, tws);
+        r2c_tile<NC, E, T, 2 * NC>(v, L, t, tws, tws);
         if (active) {
             float2 *__restrict__ x = a.X[gi] + row * nc1;
             for (int k = t; k <= NC; k += T) gstore(x, k, L[k]);
@@ -1306,14 +1424,19 @@ bool fused_supported(int sx, int sy, int sz)
     return sx >= 16 && sx <= 4096 && sy >= 8 && sy <= 4096;
 }
 
-// wide row kernels (more than 3 real fields) for the sizes such systems use
+// wide row kernels (more than 3 real fields) for the sizes such systems use;
+// narrow ones (at most 2: fewer registers, one more wave per SIMD) for the
+// sizes of large instances (others run the 3-field kernel)
 #define FUSED_WIDE_NC(NC) ((NC) >= 32 && (NC) <= 1024)
+#define FUSED_NARROW_NC(NC) ((NC) >= 32 && (NC) <= 1024)
+
+int fused_row_width(int nreal) { return nreal <= 2 ? 2 : (nreal == 3 ? 3 : FUSED_MAX_REAL); }
 
 bool fused_shapes_exist(int sx, int sy, int sz, int nreal)
 {
     const int nc = sx / 2;
     if (sz > 1) return nreal <= 3 || (nc >= 32 && nc <= 1024);
-    return !fused_row_shapes(nc, sy, nreal > 3).empty() && !fused_col_shapes(sy).empty();
+    return !fused_row_shapes(nc, sy, fused_row_width(nreal)).empty() && !fused_col_shapes(sy).empty();
 }
 
 template <int NC, int MAXR>
@@ -1327,12 +1450,13 @@ static std::vector<int> row_shapes(int ny)
     return v;
 }
 
-std::vector<int> fused_row_shapes(int nc, int ny, bool wide)
+std::vector<int> fused_row_shapes(int nc, int ny, int width)
 {
     switch (nc) {
-#define X(NC)                                                                                       \
-    case NC:                                                                                        \
-        if (wide) return FUSED_WIDE_NC(NC) ? row_shapes<NC, FUSED_MAX_REAL>(ny) : std::vector<int>(); \
+#define X(NC)                                                                                            \
+    case NC:                                                                                             \
+        if (width > 3) return FUSED_WIDE_NC(NC) ? row_shapes<NC, FUSED_MAX_REAL>(ny) : std::vector<int>(); \
+        if (width == 2 && FUSED_NARROW_NC(NC)) return row_shapes<NC, 2>(ny);                              \
         return row_shapes<NC, 3>(ny);
         FUSED_ROW_CASES(X)
 #undef X
@@ -1382,19 +1506,21 @@ static hipError_t frows_n(int lpb, const FusedInst *inst, const FusedBlock *bloc
     return hipErrorInvalidValue;
 }
 
-hipError_t fused_rows_launch(int nc, int lpb, bool wide, const FusedInst *inst, const FusedBlock *blocks,
+hipError_t fused_rows_launch(int nc, int lpb, int width, const FusedInst *inst, const FusedBlock *blocks,
                              int nblocks, uint32_t step_rel, hipStream_t s)
 {
     const float2 *sets = twiddle_sets(s);
     if (!sets) return hipErrorOutOfMemory;
     switch (nc) {
-#define X(NC)                                                                                 \
-    case NC:                                                                                  \
-        if (wide) {                                                                           \
-            if constexpr (FUSED_WIDE_NC(NC))                                                  \
+#define X(NC)                                                                                      \
+    case NC:                                                                                       \
+        if (width > 3) {                                                                           \
+            if constexpr (FUSED_WIDE_NC(NC))                                                       \
                 return frows_n<NC, FUSED_MAX_REAL>(lpb, inst, blocks, nblocks, step_rel, s, sets); \
-            return hipErrorInvalidValue;                                                      \
-        }                                                                                     \
+            return hipErrorInvalidValue;                                                           \
+        }                                                                                          \
+        if constexpr (FUSED_NARROW_NC(NC))                                                         \
+            if (width == 2) return frows_n<NC, 2>(lpb, inst, blocks, nblocks, step_rel, s, sets);  \
         return frows_n<NC, 3>(lpb, inst, blocks, nblocks, step_rel, s, sets);
         FUSED_ROW_CASES(X)
 #undef X
@@ -1446,9 +1572,9 @@ hipError_t fused_cols_launch(int ny, int twd, int stored, const FusedInst *inst,
     }
 }
 
-bool fused_small_class(int nc, int ny, bool wide)
+bool fused_small_class(int nc, int ny, int width)
 {
-    return !wide && ny == 2 * nc && (ny == 64 || ny == 128);
+    return width <= 3 && ny == 2 * nc && (ny == 64 || ny == 128);
 }
 
 template <int NC, int N>

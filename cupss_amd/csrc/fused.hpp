@@ -147,10 +147,12 @@ bool fused_shapes_exist(int sx, int sy, int sz, int nreal);
 // row kernel (nc = sx/2; wide = more than 3 real fields) and columns per
 // workgroup of the column kernel (ny = sy).  Any choice gives bit-identical
 // results.
-std::vector<int> fused_row_shapes(int nc, int ny, bool wide);
+// row kernel width (real fields held at once) for a system with nreal
+int fused_row_width(int nreal);
+std::vector<int> fused_row_shapes(int nc, int ny, int width);
 std::vector<int> fused_col_shapes(int ny);
 
-hipError_t fused_rows_launch(int nc, int lpb, bool wide, const FusedInst *inst, const FusedBlock *blocks,
+hipError_t fused_rows_launch(int nc, int lpb, int width, const FusedInst *inst, const FusedBlock *blocks,
                              int nblocks, uint32_t step_rel, hipStream_t s);
 // mode 0: prologue (real fields of the current state), 1: step
 // (ts: null, or FUSED_TS_SLOTS pairs {earliest start, latest end} of
@@ -165,7 +167,7 @@ hipError_t fused_cols_launch(int ny, int twd, int stored, const FusedInst *inst,
 // (64^2, 128^2): k steps [n0, n0+k) of the instances ids[0..nids) in one
 // launch (step n reads spectrum buffer n&1, noise counter offset n - rel0);
 // stored and ts as for fused_cols_launch (one slot pair for the launch).
-bool fused_small_class(int nc, int ny, bool wide);
+bool fused_small_class(int nc, int ny, int width);
 hipError_t fused_small_launch(int nc, int ny, int stored, const FusedInst *inst, const int *ids, int nids, int n0,
                               int k, int rel0, hipStream_t s, unsigned long long *ts = nullptr);
 // the slots of n timed launches to {~0, 0} (a kernel, so that graphs can hold it)

@@ -64,15 +64,32 @@ const float2 *twiddle_sets(hipStream_t s)
     std::lock_guard<std::mutex> lock(g_tw_mutex);
     if ((int)g_tws.size() <= dev) g_tws.resize(dev + 1, nullptr);
     if (!g_tws[dev]) {
-        std::vector<float2> h(2 * TW_N, make_float2(0.f, 0.f));
+        std::vector<float2> h(TW_SETS_LEN, make_float2(0.f, 0.f));
+        auto w = [](long num, long den) {
+            const double a = -2.0 * M_PI * (double)num / (double)den;
+            return make_float2((float)std::cos(a), (float)std::sin(a));
+        };
         for (int M = 1; M <= TW_N; M *= 2)
-            for (int j = 0; j < M; j++) {
-                double a = -2.0 * M_PI * (double)j / (double)M;
-                h[M + j] = make_float2((float)std::cos(a), (float)std::sin(a));
+            for (int j = 0; j < M; j++) h[M + j] = w(j, M);
+        // stage-ordered tables (fft_core.hpp, TW_STAGE) of every (N, E)
+        for (int E = 2; E <= 16; E *= 2)
+            for (int N = 2; N <= TW_N; N *= 2) {
+                float2 *st = h.data() + tw_stage_offset(N, E);
+                for (int NS = tw_stage_radix(N, E, 1); NS < N; NS *= tw_stage_radix(N, E, NS)) {
+                    const int R = tw_stage_radix(N, E, NS);
+                    float2 *sb = st + tw_stage_base(N, E, NS);
+                    for (int k = 0; k < NS; k++) {
+                        if (R >= 4) {
+                            for (int r = 0; r < R; r++) sb[2 * ((r / 2) * NS + k) + (r & 1)] = w((long)r * k, (long)NS * R);
+                        } else {
+                            sb[k] = w(k, 2L * NS);
+                        }
+                    }
+                }
             }
         float2 *d = nullptr;
-        if (hipMalloc(&d, sizeof(float2) * 2 * TW_N) != hipSuccess) return nullptr;
-        if (hipMemcpy(d, h.data(), sizeof(float2) * 2 * TW_N, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+        if (hipMalloc(&d, sizeof(float2) * TW_SETS_LEN) != hipSuccess) return nullptr;
+        if (hipMemcpy(d, h.data(), sizeof(float2) * TW_SETS_LEN, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
         g_tws[dev] = d;
     }
     (void)s;
