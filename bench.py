@@ -188,8 +188,8 @@ def _batch_setup(batch, args, min_s=0.3):
 # counters on a 256 MiB device copy (profiles/hbm_traffic_pmc.md has the
 # commands and the per-pass figures); keyed by workload and kernel template
 PMC_TRAFFIC = {
-    ("ensemble", "k_fcols<512,"): 1.49e8 + 1.70e8,   # 69 KPZ instances of 512^2 (seed 0)
-    ("ch2d", "k_fcols<1024,"): 8.89e6 + 6.32e6,
+    ("ensemble", "k_fcols<512,"): 1.47e8 + 1.70e8,   # 69 KPZ instances of 512^2 (seed 0)
+    ("ch2d", "k_fcols<1024,"): 8.79e6 + 6.31e6,
     ("ch3d", "k_f3z<256,"): 1.56e8 + 1.00e8,
 }
 
