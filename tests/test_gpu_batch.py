@@ -4,6 +4,8 @@
   parameters ends bit-identical to the same system advanced alone (same
   kernels, same per-system parameters and noise streams).
 * hipGraph replay equals direct launches bit for bit.
+* The full 512-instance ensemble (configuration C4): sampled members of
+  every size and model equal the same systems advanced alone.
 * Edwards-Wilkinson: the stationary structure factor of the scheme,
   S(q) = <|h_q|^2>/N = D / (dV nu q^2 (1 + dt nu q^2 / 2)), is met by both
   noise draws (real-space Philox + R2C, and the spectral-space draw).
@@ -230,6 +232,57 @@ def test_kpz_structure_factor_matches_oracle_ensemble(noise_mode):
         rb = got[band].sum() / ref[band].sum()
         print("  shell %.1f < q^2 <= %.1f (%d modes): %.4f" % (lo, hi, band.sum(), rb))
         assert abs(rb - 1.0) < 0.03
+
+
+def full_ensemble_specs(count=512, seed=0):
+    """Configuration C4: 512 independent 2D instances, side uniform in
+    {64, 128, 256, 512}; Edwards-Wilkinson at even index (lambda = 0), KPZ at
+    odd index (lambda ~ U[0, 2]); nu ~ U[0.5, 2], D = 0.01, dt = 0.005, h0 = 0."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(count):
+        n = int(rng.choice([64, 128, 256, 512]))
+        nu = float(rng.uniform(0.5, 2.0))
+        lam = float(rng.uniform(0.0, 2.0)) if i % 2 == 1 else 0.0
+        specs.append(spec_kpz(n, nu=nu, lam=lam, D=0.01, dt=0.005, seed=1000 + i))
+    return specs
+
+
+def test_full_ensemble_members_equal_alone():
+    """Configuration C4 at its full size: all 512 instances in one batch
+    (hipGraphs of 20 steps, spectral noise), 40 steps in two runs; two EW
+    and two KPZ members of every size then equal the same systems advanced
+    alone, bit for bit, and every member's state is finite."""
+    specs = full_ensemble_specs()
+    batch = C.Batch(graph_steps=20)
+    members = []
+    for sp in specs:
+        g, ids = make(sp, 1)
+        batch.add(g)
+        members.append((g, ids))
+    print("C4 ensemble: %d members, 40 steps" % len(members), flush=True)
+    batch.step(20)
+    batch.step(20)
+    picked = []
+    for n in (64, 128, 256, 512):
+        for kpz in (False, True):
+            idx = [i for i, sp in enumerate(specs) if sp["dims"][0] == n and (len(sp["terms"]) > 3) == kpz]
+            picked += idx[:2]
+    for i in picked:
+        g, ids = members[i]
+        got = g.get_real(ids["h"])
+        a, ia = make(specs[i], 1)
+        a.step(40)
+        ref = a.get_real(ia["h"])
+        a.close()
+        print("  member %3d (%d^2, %s): max|h| %.3e" % (i, specs[i]["dims"][0], "KPZ" if len(specs[i]["terms"]) > 3 else "EW",
+                                                     float(np.max(np.abs(got)))), flush=True)
+        assert np.array_equal(got, ref), i
+    for g, ids in members[::37]:
+        assert np.all(np.isfinite(g.get_real(ids["h"])))
+    batch.close()
+    for g, _ in members:
+        g.close()
 
 
 def test_model_h_split_runs_match_one_run():

@@ -9,7 +9,8 @@
   fields written back after a run.
 * Split runs and one step per call equal one run bit for bit (prologue
   equivalence, and runs that continue from the previous run's real fields).
-* Mass conservation at the full 256^3 size (every CH term carries q^2).
+* Mass conservation at the full 256^3 size (every CH term carries q^2), and
+  the oracle's FP32 scheme at that size over a few steps.
 * A noisy 3D system in real-space noise mode runs the general program.
 """
 import numpy as np
@@ -135,6 +136,24 @@ def test_cahn_hilliard_256_cubed_mass_and_general():
     err = rel_l2(x, b.get_real(ib["phi"]))
     print("  mean %.9e -> %.9e, fused-vs-general %.3e" % (m0, m1, err), flush=True)
     assert abs(m1 - m0) <= 1e-6 * max(1.0, float(np.sqrt(np.mean(x.astype(np.float64) ** 2))))
+    assert err <= 1e-5
+
+
+def test_cahn_hilliard_256_cubed_against_oracle():
+    """Configuration C3 at its full size: 256^3 Cahn-Hilliard on the fused
+    3D path against the oracle's FP32 scheme, 3 steps (about 20 s of oracle
+    work on one host core)."""
+    spec = spec_cahn_hilliard((256, 256, 256), seed=2)
+    print("256^3 CH: fused vs oracle, 3 steps", flush=True)
+    g, gid = gpu(spec, profile=True)
+    g.step(3)
+    assert_fused_3d(g)
+    got = g.get_real(gid["phi"])
+    g.close()
+    o, oid = build_system(O.System, O.pres, dict(spec, precision="f32"))
+    o.step(3)
+    err = rel_l2(got, o.get_real(oid["phi"]))
+    print("  gpu-vs-f32 %.3e" % err, flush=True)
     assert err <= 1e-5
 
 

@@ -161,6 +161,26 @@ def test_model_h(path):
     compare(spec_model_h(64), 50, path=path, expect_fused=True if path == 0 else None)
 
 
+def test_model_h_1024_against_oracle():
+    """Model H at the served configuration's largest size, 1024^2 (fused
+    path: stored stresses and velocities), 6 steps against the oracle's FP32
+    scheme (about 13 s of oracle work on one host core): phi, the velocity
+    and the stresses, whose one- and two-step lags are all in effect."""
+    spec = spec_model_h(1024, seed=17)
+    print("parity model H 1024^2, 6 steps", flush=True)
+    g, gid = gpu(spec)
+    assert g.fused_eligible
+    g.step(6)
+    got = {k: g.get_real(gid[k]) for k in ("phi", "vx", "vy", "sigxy")}
+    g.close()
+    o, oid = orc(spec)
+    o.step(6)
+    for k, a in got.items():
+        err = rel_l2(a, o.get_real(oid[k]))
+        print("  %-6s gpu-vs-f32 %.3e" % (k, err), flush=True)
+        assert err <= TOL, k
+
+
 def test_constraint_division_noise_and_nyquist():
     x0 = (np.random.default_rng(3).standard_normal(256) + 0.5).astype(np.float32)
     spec = {"dims": (16, 16, 1), "dt": 0.05,

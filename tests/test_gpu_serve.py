@@ -1,7 +1,8 @@
 """Serving (cupss_server_* through the Python Server binding): jobs of
 different sizes and models (one or two dynamic fields) arrive over time, are admitted by memory budget
 and advance together with continuous batching.  Checked here: each job's
-result is bit-identical to advancing its system alone, admission keeps
+result is bit-identical to advancing its system alone (also for model-H
+jobs at the served configuration's sizes, 128^2 to 1024^2), admission keeps
 arrival order and the budget, and the general step program replayed from a
 hipGraph equals direct launches.
 """
@@ -69,6 +70,30 @@ def test_served_results_equal_alone(quantum, max_active):
         assert np.array_equal(results[name], ref), name
     for a, b, f in times:
         assert 0 <= a <= b <= f
+
+
+def test_full_size_model_h_stream_equals_alone():
+    """Configuration C5's job mix at its sizes: model-H jobs of 128^2,
+    256^2, 512^2 and 1024^2 (and a second 128^2 and 512^2) arriving over
+    time, served with continuous batching under a budget that holds only
+    part of them at once; each result equals the job's system advanced
+    alone, bit for bit."""
+    jobs = [("mh128", spec_model_h(128, seed=51), 120), ("mh1024", spec_model_h(1024, seed=52), 30),
+            ("mh256", spec_model_h(256, seed=53), 90), ("mh512", spec_model_h(512, seed=54), 60),
+            ("mh128b", spec_model_h(128, seed=55), 75), ("mh512b", spec_model_h(512, seed=56), 40)]
+    est = []
+    for _, spec, _ in jobs:
+        sv, _ = build(spec)
+        est.append(sv.device_bytes())
+        sv.close()
+    budget = max(est) + sorted(est)[-2]
+    results, times, stats, _ = serve(jobs, budget=budget, quantum=10, max_active=8, gap=0.002)
+    print("served %d model-H jobs (128..1024) in %.3f s, peak active %d, peak %d B of %d" %
+          (len(jobs), stats["wall_s"], stats["peak_active"], stats["peak_bytes"], budget))
+    assert stats["peak_bytes"] <= budget
+    for name, spec, n in jobs:
+        ref = alone(spec, n)
+        assert np.array_equal(results[name], ref), name
 
 
 def test_admission_keeps_budget_and_arrival_order():
