@@ -96,9 +96,9 @@ def kpz_system(S, P, n, nu, lam, D=0.01, dt=0.005, seed=0, **kw):
 
 WORKLOADS = {
     "ch2d": dict(build=cahn_hilliard, dims=(1024, 1024, 1), name="2D Cahn-Hilliard 1024^2 (single instance)",
-                 cpu_steps=30),
+                 cpu_steps=50),
     "ch3d": dict(build=cahn_hilliard, dims=(256, 256, 256), name="3D Cahn-Hilliard 256^3 (single instance)",
-                 cpu_steps=1),
+                 cpu_steps=3),
     "diffusion": dict(build=diffusion, dims=(256, 256, 1), name="2D diffusion 256^2 (reference example 01)",
                       cpu_steps=2000),
     "ensemble": dict(name="batched EW/KPZ ensemble, 512 instances of 64^2..512^2", cpu_steps=0),
