@@ -57,7 +57,7 @@ struct FRowCfg {
     static constexpr int RS = lds_len(NC);  // tile row stride (>= NC + 1), odd for NC >= 32
     static constexpr bool valid()
     {
-        return THREADS >= 16 && THREADS <= 512 && (LPB * RS + 2 * NC + tw_stage_size(NC, E)) * 8 <= 150 * 1024;
+        return THREADS >= 16 && THREADS <= 512 && (LPB * RS + 2 * NC + tw_stage_alloc(NC, E)) * 8 <= 150 * 1024;
     }
 };
 
@@ -851,7 +851,7 @@ struct FSmallCfg {
     static constexpr bool valid()
     {
         return FRowCfg<NC, LPB, 3>::valid() && FColCfg<N, TWD>::valid() && LPB <= N &&
-               (2 * NC + tw_stage_size(NC, FRowCfg<NC, LPB, 3>::E) + tw_stage_size(N, FColCfg<N, TWD>::E) + LDSN) *
+               (2 * NC + tw_stage_alloc(NC, FRowCfg<NC, LPB, 3>::E) + tw_stage_alloc(N, FColCfg<N, TWD>::E) + LDSN) *
                        8 <=
                    150 * 1024;
     }
