@@ -411,8 +411,8 @@ def main():
                     help="first copy MIB MiB device-to-device (the known-size kernel that HBM counters are scaled by)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one MI355X: this build runs on a single GPU (--gpus 1); "
-              "multi-GPU sharding is a design note only.", file=sys.stderr)
+        print("bench.py measures one MI355X: this library runs on a single GPU (--gpus 1); "
+              "sharding across GPUs is not implemented.", file=sys.stderr)
         sys.exit(2)
 
     # the all-cores CPU sample spawns worker processes: before this process

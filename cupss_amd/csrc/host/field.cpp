@@ -25,7 +25,7 @@ void abi_check(int rc, const char *what)
 [[noreturn]] void per_field_step(const char *fn)
 {
     std::cout << "field::" << fn
-              << ": per-field stepping is not available in this build; evolver::advanceTime advances the "
+              << ": per-field stepping is not available in this library; evolver::advanceTime advances the "
                  "whole system on the device"
               << std::endl;
     std::exit(1);
@@ -186,7 +186,7 @@ void field::copyDeviceToHost()
 
 void field::toComp() { copyHostToDevice(); }
 void field::toReal() { copyRealDeviceToHost(); }
-void field::normalize() {}   // the inverse transform of this build is normalised
+void field::normalize() {}   // the inverse transform of this library is normalised
 void field::dealias() {}     // the mask is applied on the device where products need it
 
 void field::setToZero()

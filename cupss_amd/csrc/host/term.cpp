@@ -78,7 +78,7 @@ int term::precomputePrefactors()
 
 int term::update()
 {
-    std::cout << "term::update: terms are evaluated by evolver::advanceTime on the device in this build"
+    std::cout << "term::update: terms are evaluated by evolver::advanceTime on the device in this library"
               << std::endl;
     std::exit(1);
 }

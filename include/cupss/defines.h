@@ -46,7 +46,7 @@ struct full_term
 // The reference's PI is a float literal (defines.h:54); the q grid uses it.
 #define PI 3.1415926535f
 
-// RUN_CPU is accepted for compatibility; this build executes every system on
+// RUN_CPU is accepted for compatibility; this library executes every system on
 // the GPU (see evolver.h).
 enum DeviceToRunOn {RUN_CPU, RUN_GPU};
 

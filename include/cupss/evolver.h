@@ -104,7 +104,7 @@ public:
     void addDroplet(std::string field, float value, float radius, float interface_width, int center_x, int center_y, int center_z);
     void initializeFromFile(std::string field, std::string file, int skiprows, char delimiter);
 
-    // Extensions of this build (not in the reference)
+    // Extensions of this library (not in the reference)
     void setSeed(uint64_t seed);               // seed of the in-kernel noise
     struct cupss_system_t *deviceSystem();     // the C-ABI handle of this system
 };

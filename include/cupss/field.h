@@ -5,7 +5,7 @@
 // authoritative until prepareProblem(); afterwards the device is, and the
 // host copies are refreshed by writeOut()/copyAllDataToHost() (and after
 // every step for RUN_CPU systems).  The *_d device pointers and the
-// precomputed tables of the reference are not used by this build (null):
+// precomputed tables of the reference are not used by this library (null):
 // spectra live on the device as half spectra and the multipliers are
 // computed in-kernel.  Callbacks keep the reference signature and receive
 // a float2[N] array of real values (device pointer on RUN_GPU systems,
@@ -111,7 +111,7 @@ class field {
     dim3 threads_per_block;
     dim3 blocks;
 
-    // The per-field step functions of the reference.  In this build a step
+    // The per-field step functions of the reference.  In this library a step
     // runs for the whole system on the device (evolver::advanceTime); these
     // keep their names for source compatibility and act on the host copies
     // or forward to the system as documented in field.cpp.
@@ -150,7 +150,7 @@ class field {
     void printImplicitString();
     int updateParameter(const std::string &, float);
 
-    // Extension of this build: id of the field in the device system.
+    // Extension of this library: id of the field in the device system.
     int deviceId() const { return device_id; }
 };
 

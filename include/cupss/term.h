@@ -1,5 +1,5 @@
 // term.h -- one right-hand-side group: a product of fields with summed
-// spectral prefactors (reference: inc/cupss/term.h:14-70).  In this build a
+// spectral prefactors (reference: inc/cupss/term.h:14-70).  In this library a
 // term is evaluated on the device by the system's step program; the public
 // members keep the reference names (device pointers and tables are null).
 #ifndef TERM_H

@@ -1,5 +1,5 @@
 // reference_cases.cpp -- the scenarios of the reference test suite
-// (tests/tests.cpp:64-422) through this build's C++ API: droplet initial
+// (tests/tests.cpp:64-422) through this library's C++ API: droplet initial
 // conditions (1D/2D/3D) and one step of the spectral operators -q^2, iqx,
 // iqy, iqz, 1/q (1D/3D), for RUN_CPU and RUN_GPU systems, against the golden
 // files.  The iqy/iqz fields are checked against their own files.

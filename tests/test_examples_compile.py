@@ -4,9 +4,9 @@ include/ and libcupss.so (drop-in check of the C++ API).
 Needs the reference tree at /root/reference (present where the library is
 developed, absent on GPU boxes); skipped elsewhere.  The sources are copied
 into a temporary tree whose inc/ points at include/, so their relative
-#include "../../inc/cupss.h" resolves to this build's headers.  Examples 03
+#include "../../inc/cupss.h" resolves to this library's headers.  Examples 03
 and 04 include <cuda.h>/<cuda_runtime.h> without using them; they are built
-with those two lines removed (no CUDA headers exist in this build)."""
+with those two lines removed (no CUDA headers exist here)."""
 import os
 import re
 import shutil
