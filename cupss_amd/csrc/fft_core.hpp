@@ -236,12 +236,14 @@ struct SizeTwiddles {
 //
 // For the transform of N points with E per lane, the stage of span NS > 1
 // and radix R needs w^(r*k) = exp(-2*pi*i*r*k/(NS*R)) for r = 1..R-1 and
-// k = j mod NS.  The stage table holds them in the order the lanes read
-// them: for R >= 4, entry 2*(q*NS + k) + p is w^((2q+p)*k), q < R/2, so a
-// lane fetches its R twiddles as R/2 aligned 16-byte pairs at one address
-// (k) plus constant offsets, and consecutive k are consecutive pairs (no
-// bank conflicts); for R = 2, entry k is w^k.  The stages' tables follow
-// each other in stage order.  The tables of every (N, E) sit in the same
+// k = j mod NS.  The stage table holds them as R-1 rows of NS: entry
+// (r-1)*NS + k is w^(r*k), so a lane fetches its R-1 twiddles at one
+// address (k) plus constant offsets, each as an 8-byte load in which
+// consecutive lanes read consecutive entries: the LDS then serves the 32
+// lanes of a half-wave from distinct bank pairs at full rate (a 16-byte
+// load per lane would be served as two 8-byte passes of stride 16 bytes,
+// half the rate).  The stages' tables follow each other in stage order,
+// N - R0 entries in all for a first radix R0.  The tables of every (N, E) sit in the same
 // device array as the per-size tables (twiddle_sets()), at tw_stage_offset.
 constexpr int TW_STAGE = 0;
 
@@ -251,7 +253,7 @@ __host__ __device__ constexpr int tw_stage_radix(int N, int E, int NS)
 }
 __host__ __device__ constexpr int tw_stage_len(int N, int E, int NS)
 {
-    return tw_stage_radix(N, E, NS) >= 4 ? tw_stage_radix(N, E, NS) * NS : NS;
+    return (tw_stage_radix(N, E, NS) - 1) * NS;
 }
 // offset of the table of the stage with span NS (its sum of the earlier
 // twiddled stages' lengths); tw_stage_base(N, E, N) is the whole size
@@ -364,17 +366,9 @@ struct Stockham {
                 const int j = t + b * T;
                 const int k = j & (NS - 1);
                 if constexpr (NS > 1 && TWS == TW_STAGE) {
-                    const float2 *__restrict__ st = tw + tw_stage_base(N, E, NS);
-                    if constexpr (R >= 4) {
+                    const float2 *__restrict__ st = tw + tw_stage_base(N, E, NS) + k;
 #pragma unroll
-                        for (int q = 0; q < R / 2; q++) {
-                            const float4 w = ((const float4 *)st)[q * NS + k];
-                            if (q > 0) a[2 * q] = cmul(a[2 * q], tw_dir<DIR>(make_float2(w.x, w.y)));
-                            a[2 * q + 1] = cmul(a[2 * q + 1], tw_dir<DIR>(make_float2(w.z, w.w)));
-                        }
-                    } else {
-                        a[1] = cmul(a[1], tw_dir<DIR>(st[k]));
-                    }
+                    for (int r = 1; r < R; r++) a[r] = cmul(a[r], tw_dir<DIR>(st[(r - 1) * NS]));
                 } else if constexpr (NS > 1) {
 #pragma unroll
                     for (int r = 1; r < R; r++) a[r] = cmul(a[r], tw_lookup<DIR, TWS>(tw, r * k, NS * R));

@@ -49,12 +49,30 @@ namespace cupss {
 // with more; those use 4 elements per lane to stay within the registers).
 // Rows of 1024 real points also take 4 per lane (more waves for one large
 // system).
+// Tile row stride: at least lds_len(nc), since each row's transform works
+// in place, and chosen for the LDS banks (32 of 4 bytes; a store is served
+// 16 lanes at a time, a load 32).  A tile copy moves row pairs: the 16
+// lanes of a store cover LPB/2 pairs, 32/(LPB/2) dwords of each, and
+// successive pairs lie 4*RS dwords apart, so the runs interleave without
+// sharing a bank when RS is odd (LPB 16), 2 mod 4 (LPB 8) or 4 mod 8 (LPB 4).
+__host__ __device__ constexpr int tile_stride(int nc, int lpb)
+{
+    int rs = lds_len(nc);
+    if (lpb == 16)
+        while (rs % 2 != 1) rs++;
+    else if (lpb == 8)
+        while (rs % 4 != 2) rs++;
+    else if (lpb == 4)
+        while (rs % 8 != 4) rs++;
+    return rs;
+}
+
 template <int NC, int LPB, int MAXR = 3>
 struct FRowCfg {
     static constexpr int E = MAXR > 3 ? (NC >= 4 ? 4 : NC) : (NC >= 1024 ? 16 : (NC >= 512 ? 4 : (NC >= 64 ? 8 : (NC >= 4 ? 4 : NC))));
     static constexpr int T = NC / E;
     static constexpr int THREADS = LPB * T;
-    static constexpr int RS = lds_len(NC);  // tile row stride (>= NC + 1), odd for NC >= 32
+    static constexpr int RS = tile_stride(NC, LPB);
     static constexpr bool valid()
     {
         return THREADS >= 16 && THREADS <= 512 && (LPB * RS + 2 * NC + tw_stage_alloc(NC, E)) * 8 <= 150 * 1024;

@@ -78,13 +78,8 @@ const float2 *twiddle_sets(hipStream_t s)
                 for (int NS = tw_stage_radix(N, E, 1); NS < N; NS *= tw_stage_radix(N, E, NS)) {
                     const int R = tw_stage_radix(N, E, NS);
                     float2 *sb = st + tw_stage_base(N, E, NS);
-                    for (int k = 0; k < NS; k++) {
-                        if (R >= 4) {
-                            for (int r = 0; r < R; r++) sb[2 * ((r / 2) * NS + k) + (r & 1)] = w((long)r * k, (long)NS * R);
-                        } else {
-                            sb[k] = w(k, 2L * NS);
-                        }
-                    }
+                    for (int r = 1; r < R; r++)
+                        for (int k = 0; k < NS; k++) sb[(r - 1) * NS + k] = w((long)r * k, (long)NS * R);
                 }
             }
         float2 *d = nullptr;
