@@ -147,10 +147,34 @@ struct Dft<4, DIR> {
     }
 };
 
-// LDS index with padding: one extra slot every 32 entries, which breaks the
-// power-of-two strides of the Stockham writes across banks.
+// LDS index with padding: one extra slot every 32 entries.  This is the
+// layout of a finished transform in LDS (and of the last Stockham stage's
+// exchange); the earlier stages use their own (stage_pad).
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 5); }
-__host__ __device__ constexpr int lds_len(int n) { return n + (n >> 5) + 1; }
+
+// Exchange layout of a Stockham stage that is not the last.  The LDS has
+// 32 banks of 4 bytes and serves an 8-byte store 16 lanes at a time, an
+// 8-byte load 32 lanes at a time.  Loads read consecutive entries, which
+// any of these layouts keeps at two dwords per bank for 32 lanes (the
+// least 64 dwords allow).
+// Stores put lane j's R outputs at (j - k)*R + k + r*NS, k = j mod NS:
+// * NS = 1 (stride R): one slot every 16 entries spreads 16 lanes over
+//   all 16 bank pairs (for R = 2, 4, 8, 16);
+// * NS = 8, R = 8 (runs of 8 lanes, 64 entries apart): 8 slots every 64
+//   entries put the two runs of a quarter-wave on opposite halves of the
+//   banks;
+// * otherwise runs of 16 or more consecutive entries: lpad.
+// Each form is i + c*floor(i/2^s), so pad(base + r*NS) = pad(base) +
+// pad(r*NS) for the bases a stage uses, as for lpad.
+template <int NS, int R>
+__device__ __forceinline__ int stage_pad(int i)
+{
+    if constexpr (NS == 1) return i + (i >> 4);
+    else if constexpr (NS == 8 && R == 8) return i + 8 * (i >> 6);
+    else return lpad(i);
+}
+// entries of one line's LDS region: the largest of the layouts (i + i/8)
+__host__ __device__ constexpr int lds_len(int n) { return n + (n >> 3) + 1; }
 
 constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
 
@@ -355,6 +379,12 @@ struct Stockham {
     static constexpr int REM = N / NS;
     static constexpr int R = REM >= E ? E : REM;
     static constexpr bool LAST = NS * R == N;
+    // the last stage leaves the transform in the lpad layout
+    __device__ __forceinline__ static int pad(int i)
+    {
+        if constexpr (LAST) return lpad(i);
+        else return stage_pad<NS, R>(i);
+    }
     __device__ __forceinline__ static void run(float2 (&v)[E], float2 *lds, int t, const float2 *__restrict__ tw)
     {
         if constexpr (NS < N) {
@@ -378,21 +408,22 @@ struct Stockham {
 #pragma unroll
                     for (int r = 0; r < R; r++) v[b + r * (E / R)] = a[r];
                 } else {
-                    // lpad(base + r*NS) = lpad(base) + lpad(r*NS): base = (j-k)*R + k
+                    // pad(base + r*NS) = pad(base) + pad(r*NS): base = (j-k)*R + k
                     // with (j-k)*R a multiple of NS*R, so the R outputs either stay
-                    // inside one 32-entry block (NS*R <= 32) or step across blocks
-                    // exactly as r*NS does.  One address register, constant offsets.
-                    const int wb = lpad((j - k) * R + k);
+                    // inside one padding block (NS*R <= its size) or step across
+                    // blocks exactly as r*NS does.  One address register, constant
+                    // offsets.
+                    const int wb = pad((j - k) * R + k);
 #pragma unroll
-                    for (int r = 0; r < R; r++) lds[wb + lpad(r * NS)] = a[r];
+                    for (int r = 0; r < R; r++) lds[wb + pad(r * NS)] = a[r];
                 }
             }
             if constexpr (!(LAST && !OUT_LDS)) {
                 line_sync<WL>();
-                // lpad(t + e*T) = lpad(t) + lpad(e*T) for t < T, T a power of two
-                const int rb = lpad(t);
+                // pad(t + e*T) = pad(t) + pad(e*T) for t < T, T a power of two
+                const int rb = pad(t);
 #pragma unroll
-                for (int e = 0; e < E; e++) v[e] = lds[rb + lpad(e * T)];
+                for (int e = 0; e < E; e++) v[e] = lds[rb + pad(e * T)];
                 if constexpr (!LAST) line_sync<WL>();
             }
             Stockham<N, E, DIR, NS * R, TWS, OUT_LDS, WL>::run(v, lds, t, tw);
