@@ -40,10 +40,39 @@ using namespace cupss;
 
 namespace cupss {
 
+// End of a run of `by` steps: the step counters advance, and after an odd
+// count each double buffer (phi's spectrum, the stored fields) swaps its
+// halves, so that [0] is again the buffer holding the current state.  The
+// host applies the same changes to its copies (mirror_run_end), so the next
+// run finds the descriptors unchanged and uploads nothing for them.
 __global__ void k_bump_steps(FusedInst *inst, int n, uint32_t by)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) inst[i].step_base += by;
+    if (i >= n) return;
+    FusedInst &a = inst[i];
+    a.step_base += by;
+    if (by & 1) {
+        float2 *t = a.spec[0];
+        a.spec[0] = a.spec[1];
+        a.spec[1] = t;
+        for (int c = 0; c < a.ncons; c++) {
+            t = a.cs[c][0];
+            a.cs[c][0] = a.cs[c][1];
+            a.cs[c][1] = t;
+        }
+    }
+}
+
+// the host side of k_bump_steps on the descriptor copies
+static void mirror_run_end(std::vector<FusedInst> &h, uint32_t by)
+{
+    for (FusedInst &a : h) {
+        a.step_base += by;
+        if (by & 1) {
+            std::swap(a.spec[0], a.spec[1]);
+            for (int c = 0; c < a.ncons; c++) std::swap(a.cs[c][0], a.cs[c][1]);
+        }
+    }
 }
 
 struct Tables {
@@ -89,9 +118,8 @@ struct FusedBatch {
     bool built = false;
 
     // the last run's transpose back to the natural layout, deferred (see
-    // settle and flush): pending, and the parity of that run's length
+    // settle and flush)
     bool settle_pending = false;
-    int settle_fin = 0;
 
     bool autotune = true;
     bool tuned = false;
@@ -110,6 +138,7 @@ struct FusedBatch {
     unsigned long long *pts = nullptr;  // timestamp slots of up to PROF_SLOTS timed launches
     int pcount = 0;
     int psteps = 1;  // steps one timed launch takes (a small-instance launch takes a run's k)
+    int gpcount = 0, gpsteps = 1;  // the same for one replay of the captured graph
     static constexpr int PROF_SLOTS = 64;
 
     // independent work (size classes, the elementwise set) runs on up to
@@ -923,7 +952,6 @@ static int settle(FusedBatch &B, int nsteps)
     // the transpose back is deferred: a following run resumes from the
     // column-major state; anything else that touches a member flushes it
     B.settle_pending = true;
-    B.settle_fin = nsteps & 1;
     bool waits = false;
     for (cupss_system s : B.sys) waits = waits || s->stream != B.stream;
     if (waits) {
@@ -964,7 +992,9 @@ static int flush(FusedBatch &B)
 {
     if (!B.settle_pending) return CUPSS_OK;
     B.settle_pending = false;
-    if (B.ntb > 0) HIP_TRY(fused_settle_launch(B.d, B.tb, B.ntb, B.settle_fin, B.stream));
+    // the run's last k_bump_steps left the final state in buffer [0] and
+    // the last step's start in [1]
+    if (B.ntb > 0) HIP_TRY(fused_settle_launch(B.d, B.tb, B.ntb, 0, B.stream));
     bool waits = false;
     for (cupss_system s : B.sys) {
         if (s->pending == &B) s->pending = nullptr;
@@ -1054,23 +1084,29 @@ static int run(FusedBatch &B, int nsteps)
             }
             B.graph = g;
             HIP_TRY(hipGraphInstantiate(&B.gexec, g, nullptr, nullptr, 0));
+            B.gpcount = B.pcount;
+            B.gpsteps = B.psteps;
         }
         while (nsteps - done >= K) {
             HIP_TRY(hipGraphLaunch(B.gexec, B.stream));
             done += K;
         }
+        // a replay clears the timestamp slots and fills them as the captured
+        // launches did (the host counted those at the capture)
+        B.pcount = B.gpcount;
+        B.psteps = B.gpsteps;
         // the remainder continues from the bumped counters
-        for (size_t i = 0; i < B.h.size(); i++) B.h[i].step_base += (uint64_t)done;
+        mirror_run_end(B.h, (uint32_t)done);
     }
     if (nsteps > done) {
         if (int r = launch_all(B, done, nsteps - done, done)) return r;
-        // the device's step counters follow the steps, so that the next
-        // run's descriptors need no upload for them
+        // the device's step counters and buffer order follow the steps, so
+        // that the next run's descriptors need no upload for them
         const int ni = (int)B.h.size();
         hipLaunchKernelGGL(k_bump_steps, dim3((ni + 255) / 256), dim3(256), 0, B.stream, B.d, ni,
                            (uint32_t)(nsteps - done));
         HIP_TRY(hipGetLastError());
-        for (size_t i = 0; i < B.h.size(); i++) B.h[i].step_base += (uint64_t)(nsteps - done);
+        mirror_run_end(B.h, (uint32_t)(nsteps - done));
     }
     B.steps_run += nsteps;
     return settle(B, nsteps);
