@@ -74,6 +74,39 @@ def test_batch_equals_alone(noise_mode, graph):
     assert not bad
 
 
+def test_odd_runs_around_graph_replays_equal_alone():
+    """Runs of odd length swap the roles of each member's two spectrum
+    buffers (and of the stored fields' pairs) on the device; graph replays,
+    direct launches and the settle that follow must find the current state
+    in the buffer they start from.  5 (replay + 1 direct), 8 (two replays
+    from the swapped pair), 3 (direct only), 7 (replay + 3 direct) steps
+    against 23 steps alone, every field of every member bit for bit."""
+    specs = mixed_specs()
+    alone = []
+    for sp in specs:
+        g, ids = make(sp, 1)
+        g.step(23)
+        alone.append({k: g.get_real(v) for k, v in ids.items()})
+    batch = C.Batch(graph_steps=4)
+    members = []
+    for sp in specs:
+        g, ids = make(sp, 1)
+        batch.add(g)
+        members.append((g, ids))
+    for n in (5, 8, 3, 7):
+        print("batch run of", n, "steps", flush=True)
+        batch.step(n)
+    bad = []
+    for i, ((g, ids), ref) in enumerate(zip(members, alone)):
+        assert g.step_count == 23
+        for k, v in ids.items():
+            got = g.get_real(v)
+            if not np.array_equal(got, ref[k]):
+                bad.append((i, k, float(np.max(np.abs(got - ref[k])))))
+    print("mismatches", bad)
+    assert not bad
+
+
 def test_split_runs_equal_one_run():
     """Advancing 10 + 13 steps, or one step per call (the C++ evolver's
     advanceTime), equals advancing 23: runs that resume from the previous
